@@ -1,0 +1,183 @@
+"""Oracle-side local Hilbert space: numpy restatement of the reference's model setup.
+
+TEST INFRASTRUCTURE ONLY.  Nothing in the product imports this module; it builds
+the block structure the C oracle (``ctqmc_oracle.c``) samples with, following
+
+* ``States.F90:58-184``   occupation basis, ``qns2substates`` blocking, offsets
+* ``Operator.F90:115-200`` ``init_Psi``: c/c+ blocks and the ``Connect`` table
+* ``Operator.F90:481-542, 1041-1109`` one-particle and U terms of H_loc
+* ``Operator.F90:1111-1124`` ``force_hermitian``
+* ``Operator.F90:637-670`` per-block ``DSYEV`` (here ``numpy.linalg.eigh``)
+* ``Operator.F90:1130-1168`` ``transform_Psis``: eigenbasis operator blocks
+* ``Trace.F90:559-568`` ``Egs`` = lowest eigenvalue (no outer truncation)
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+
+def _nt(state: int, n: int) -> int:
+    return bin(state).count("1")
+
+
+def _szt2(state: int, n: int) -> int:
+    """2*Sz with the reference's convention: bits 0..N-1 count -1/2
+    (``States.F90:224-234``)."""
+    lo = bin(state & ((1 << n) - 1)).count("1")
+    hi = bin(state >> n).count("1")
+    return hi - lo
+
+
+class OracleModel:
+    """Block-diagonal local Hamiltonian in its eigenbasis."""
+
+    def __init__(self, umatrix: np.ndarray, muimp: np.ndarray, quantum_numbers: str = "Nt Szt"):
+        n = umatrix.shape[0]
+        self.nbands = n
+        nstates = 4 ** n
+        self.nstates = nstates
+        # qns2substates (States.F90:151-184): first-come order over state indices
+        s2s = -np.ones(nstates, dtype=int)
+        nsst = 0
+        for st1 in range(nstates):
+            if s2s[st1] != -1:
+                continue
+            s2s[st1] = nsst
+            for st2 in range(st1, nstates):
+                if s2s[st2] != -1:
+                    continue
+                if "Nt" in quantum_numbers and _nt(st1, n) != _nt(st2, n):
+                    continue
+                if "Szt" in quantum_numbers and _szt2(st1, n) != _szt2(st2, n):
+                    continue
+                s2s[st2] = nsst
+            nsst += 1
+        self.nsst = nsst
+        self.states = [np.where(s2s == s)[0] for s in range(nsst)]
+        self.sizes = np.array([len(x) for x in self.states])
+        self.offsets = np.concatenate([[0], np.cumsum(self.sizes)[:-1]])
+        self.state_sst = s2s
+
+        # c/c+ blocks in the occupation basis (Operator.F90:115-200)
+        # connect[s, b, spin, ca] with ca 0 = creator, 1 = annihilator
+        connect = -np.ones((nsst, n, 2, 2), dtype=int)
+        psi = {}
+        for s1 in range(nsst):
+            for b in range(n):
+                for sp in range(2):
+                    bit = b + n * sp
+                    for ca in range(2):
+                        mat = {}
+                        for i1, st in enumerate(self.states[s1]):
+                            occ = (st >> bit) & 1
+                            if (ca == 0 and occ) or (ca == 1 and not occ):
+                                continue
+                            t = st ^ (1 << bit)
+                            sign = -1.0 if bin(st & ((1 << bit) - 1)).count("1") % 2 else 1.0
+                            s2 = s2s[t]
+                            mat.setdefault(s2, []).append((i1, t, sign))
+                        if not mat:
+                            continue
+                        assert len(mat) == 1, "operator maps one block to several"
+                        (s2, ent), = mat.items()
+                        connect[s1, b, sp, ca] = s2
+                        m = np.zeros((self.sizes[s2], self.sizes[s1]))
+                        pos = {st: i for i, st in enumerate(self.states[s2])}
+                        for i1, t, sign in ent:
+                            m[pos[t], i1] = sign
+                        psi[(s1, b, sp, ca)] = m
+        self.connect = connect
+
+        # H_loc blocks
+        h = [np.zeros((k, k)) for k in self.sizes]
+        for b1 in range(n):
+            for sp1 in range(2):
+                for b2 in range(n):
+                    for sp2 in range(2):
+                        v = muimp[b1, sp1, b2, sp2]
+                        if v == 0:
+                            continue
+                        for s in range(nsst):
+                            s2 = connect[s, b2, sp2, 1]
+                            if s2 < 0:
+                                continue
+                            s1 = connect[s2, b1, sp1, 0]
+                            if s1 != s:
+                                continue
+                            h[s] += v * psi[(s2, b1, sp1, 0)] @ psi[(s, b2, sp2, 1)]
+        um = umatrix.reshape((n, 2) * 4)
+        for idx in np.argwhere(um != 0):
+            b1, sp1, b2, sp2, b3, sp3, b4, sp4 = idx
+            v = 0.5 * um[tuple(idx)]
+            for s in range(nsst):
+                s4 = connect[s, b3, sp3, 1]
+                if s4 < 0:
+                    continue
+                s3 = connect[s4, b4, sp4, 1]
+                if s3 < 0:
+                    continue
+                s2 = connect[s3, b2, sp2, 0]
+                if s2 < 0:
+                    continue
+                s1 = connect[s2, b1, sp1, 0]
+                if s1 != s:
+                    continue
+                h[s] += v * (psi[(s2, b1, sp1, 0)] @ psi[(s3, b2, sp2, 0)]
+                             @ psi[(s4, b4, sp4, 1)] @ psi[(s, b3, sp3, 1)])
+        self.hblocks = [0.5 * (x + x.T) for x in h]
+
+        self.evals = []
+        self.evecs = []
+        for x in self.hblocks:
+            w, v = np.linalg.eigh(x)
+            self.evals.append(w)
+            self.evecs.append(v)
+        self.egs = min(w[0] for w in self.evals)
+
+        # eigenbasis operator blocks
+        self.psi_eb = {}
+        for (s1, b, sp, ca), m in psi.items():
+            s2 = connect[s1, b, sp, ca]
+            self.psi_eb[(s1, b, sp, ca)] = self.evecs[s2].T @ m @ self.evecs[s1]
+        self.psi_occ = psi
+
+    # ------------------------------------------------------------------
+    def flat(self) -> dict:
+        """Arrays handed to the C oracle (``oracle_create``)."""
+        n = self.nbands
+        nsst = self.nsst
+        nmax = int(self.sizes.max())
+        evals = np.zeros(int(self.sizes.sum()))
+        for s in range(nsst):
+            evals[self.offsets[s]:self.offsets[s] + self.sizes[s]] = self.evals[s]
+        # operator blocks: for each (s, b, sp, ca) with a target, a dense
+        # column-major (n_target x n_source) block; opoff = -1 if none
+        opoff = -np.ones((nsst, n, 2, 2), dtype=np.int64)
+        chunks = []
+        pos = 0
+        for s in range(nsst):
+            for b in range(n):
+                for sp in range(2):
+                    for ca in range(2):
+                        m = self.psi_eb.get((s, b, sp, ca))
+                        if m is None:
+                            continue
+                        opoff[s, b, sp, ca] = pos
+                        chunks.append(np.asfortranarray(m).ravel(order="F"))
+                        pos += m.size
+        ops = np.concatenate(chunks) if chunks else np.zeros(1)
+        evec = np.zeros(int((self.sizes ** 2).sum()))
+        evoff = np.zeros(nsst, dtype=np.int64)
+        p = 0
+        for s in range(nsst):
+            evoff[s] = p
+            evec[p:p + self.sizes[s] ** 2] = np.asfortranarray(self.evecs[s]).ravel(order="F")
+            p += self.sizes[s] ** 2
+        first_state = np.array([self.states[s][0] for s in range(nsst)], dtype=np.int32)
+        return dict(nsst=nsst, nmax=nmax, sizes=self.sizes.astype(np.int32),
+                    offsets=self.offsets.astype(np.int32), evals=evals, egs=self.egs,
+                    connect=self.connect.astype(np.int32), opoff=opoff, ops=ops,
+                    evec=evec, evoff=evoff, first_state=first_state,
+                    state_sst=self.state_sst.astype(np.int32),
+                    states=np.concatenate(self.states).astype(np.int32))
