@@ -1,0 +1,134 @@
+/*
+ * C-ABI of the MI355X CT-HYB sampling core.
+ *
+ * This is the drop-in boundary for the Z-space sampling path of the reference
+ * solver module MCTQMC (w2dynamics src/ctqmc_fortran/CTQMC.F90).  Each entry point
+ * names the reference interface it replaces.  Conventions:
+ *   - plain pointers and sizes only; arrays are column-major (Fortran order) as
+ *     the reference receives them through f2py;
+ *   - every call returns an int status (W2D_OK = 0), nothing is thrown across
+ *     the boundary; the reference stops the process instead (MatrixUpdate.F90:69-98,
+ *     CTQMC.F90:4062, Trace.F90:668);
+ *   - one handle owns a batch of independent Markov chains on one GPU; chain c
+ *     is seeded with seed0 + c, so chain 0 reproduces the reference's stream for
+ *     init_rnd(seed0) and the batch plays the role of MPI ranks NSeed + rank
+ *     (cthyb:144);
+ *   - a handle is not thread-safe; one host thread drives it.
+ */
+#ifndef W2DYN_CTQMC_H
+#define W2DYN_CTQMC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define W2D_OK 0
+#define W2D_ERR_ARG 1          /* invalid argument or unsupported parameter value */
+#define W2D_ERR_HIP 2          /* HIP runtime error (allocation, launch) */
+#define W2D_ERR_CAPACITY 3     /* expansion order or window length above the capacity */
+#define W2D_ERR_SINGULAR 4     /* "DetRat too small in removal" (CTQMC.F90:4062) */
+#define W2D_ERR_ZERO_TRACE 5   /* "Zero trace initial state" (Trace.F90:668) */
+#define W2D_ERR_STATE 6        /* call out of order (e.g. results before measure) */
+
+typedef struct w2d_ctqmc w2d_ctqmc;
+
+/* Per-move record used by the parity tests; the same fields the reference driver
+ * reads after each Step* call (CTQMC.F90:3353-4835). */
+typedef struct {
+    int32_t kind;      /* 1 add, 2 remove, 3 global, 4 tau shift */
+    int32_t outcome;   /* 0 rejected, 1 accepted, 2 rejected before the weight (QN / no pair) */
+    int32_t noper;     /* operators in the configuration after the move */
+    int32_t outer_sst; /* outer superstate after the move */
+    int32_t window;    /* sliding-window position after the move */
+    int32_t pad;
+    double trace_log, trace_sign, det_log, det_sign, cfgsign;
+} w2d_move_record;
+
+/* Replaces init_paras (CTQMC.F90:9412) + init_solver (CTQMC.F90:7894) + init_rnd
+ * (CTQMC.F90:9404) for a batch of chains.
+ *   params     : the "key = value #\n" string (Parameters.F90:281-320)
+ *   u_matrix   : (2N,2N,2N,2N), flavour index 2*(b-1)+s   (CTQMC.F90:7899)
+ *   ftau_full  : (N,2,N,2,nftau), only the flavour-diagonal part is used
+ *   muimp_full : (N,2,N,2)
+ *   capacity   : maximum pairs per flavour a chain may reach (0 = default 256)
+ * Screening (Uw) is not supported; offdiag, segment, statesampling, worm and
+ * four-operator moves must be off. */
+int w2d_ctqmc_create(const char *params, int nbands, int nftau, const double *u_matrix,
+                     const double *ftau_full, const double *muimp_full, int nchains,
+                     uint32_t seed0, int capacity, int device, w2d_ctqmc **out);
+
+/* Replaces set_taudiffmax (CTQMC.F90:8180). */
+int w2d_ctqmc_set_taudiffmax(w2d_ctqmc *h, double taudiff_max);
+
+/* Replaces ctqmc_warmup (CTQMC.F90:8188): nwarmups Metropolis steps per chain. */
+int w2d_ctqmc_warmup(w2d_ctqmc *h, int64_t nwarmups);
+
+/* Replaces ctqmc_measure(1,1) (CTQMC.F90:8355): nmeas x (NCorr steps + Z-space
+ * measurements) per chain; accumulates into the handle's per-chain buffers,
+ * which are cleared first (init_counters, CTQMC.F90:847). */
+int w2d_ctqmc_measure(w2d_ctqmc *h, int64_t nmeas);
+
+/* Sizes of the result arrays. */
+int w2d_ctqmc_dims(const w2d_ctqmc *h, int *nbands, int *nchains, int *ntau, int *nlegmax,
+                   int *nsst, int *nstates, int *maxhisto);
+
+/* Raw per-chain accumulators after measure, before the normalisation of
+ * CTQMC.F90:8663-8771 (which the host applies, so that per-chain samples give
+ * the across-chain error of DistributedSample, statistics.py:180-209).
+ * Any pointer may be NULL.  Shapes (C order, chain slowest):
+ *   gtau   [nchains][2N][ntau]    sum over measurements of the binned M^{-1}
+ *   gleg   [nchains][2N][nlegmax]
+ *   dm     [nchains][nstates][nstates]  beta/2 density matrix, eigenbasis,
+ *          block-diagonal, column-major within a chain
+ *   histo  [nchains][2N][maxhisto+1]
+ *   outer  [nchains][nsst]
+ *   scalars[nchains][12]: sum cfgsign, measurements, acc/try add, rem, glob, shift,
+ *          attempted moves, qn-rejected moves
+ * flavour index f = b + N*s (orbital fastest). */
+int w2d_ctqmc_get_accumulators(w2d_ctqmc *h, double *gtau, double *gleg, double *dm,
+                               double *histo, double *outer, double *scalars);
+
+/* Eigenbasis of the local Hamiltonian built by create: per-superstate sizes and
+ * offsets, eigenvalues (EigenstatesEnergies), occupation-basis state of each
+ * index (States), and the block eigenvectors (column-major, concatenated);
+ * needed to transform the density matrix to the occupation basis
+ * (Transform_DensityMatrix_EB_to_OCCB, CTQMC.F90:3163). */
+int w2d_ctqmc_get_model(const w2d_ctqmc *h, int *sizes, int *offsets, double *evals,
+                        int *states, double *evecs, double *egs);
+
+/* Parity entry point: run n steps of chain `chain` alone (other chains idle) and
+ * record one w2d_move_record per step. */
+int w2d_ctqmc_moves_log(w2d_ctqmc *h, int chain, int64_t n, w2d_move_record *rec);
+
+/* Replaces get_mc_config_scalars / get_mc_config (CTQMC.F90:9358-9376) for one
+ * chain: returns the configuration (0-based orbital/spin, ca 1 = creator,
+ * 2 = annihilator), weights[5] = trace log/sign, det log/sign, cfgsign. */
+int w2d_ctqmc_get_config(w2d_ctqmc *h, int chain, int cap, int *n, double *tau, int *orb,
+                         int *spin, int *ca, int *outer_sst, double *weights);
+
+/* Replaces set_mc_config (CTQMC.F90:9378) for one chain: full trace, full
+ * inverse hybridisation matrices and sign of the given configuration. */
+int w2d_ctqmc_set_config(w2d_ctqmc *h, int chain, int n, const double *tau, const int *orb,
+                         const int *spin, const int *ca, int outer_sst, double *weights);
+
+/* Inverse hybridisation block of flavour f of one chain (column-major k x k,
+ * rows creators, columns annihilators as Trace.F90:2279-2333); *k receives the
+ * order, out may be NULL to query it. */
+int w2d_ctqmc_get_minv(w2d_ctqmc *h, int chain, int f, int *k, double *out);
+
+/* Kernel time of the last warmup/measure call in milliseconds (HIP events on
+ * the handle's stream). */
+int w2d_ctqmc_last_kernel_ms(const w2d_ctqmc *h, double *ms);
+
+/* Replaces dest_ctqmc (CTQMC.F90:901). */
+int w2d_ctqmc_destroy(w2d_ctqmc *h);
+
+/* Library/ABI version string. */
+const char *w2d_ctqmc_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,58 @@
+// Device-side data layout of the batched CT-HYB engine.
+#pragma once
+
+#include <cstdint>
+
+namespace w2d {
+
+constexpr int WAVE = 64;       // one wavefront drives one Markov chain
+constexpr int MAXF = 14;       // flavours (2*nbands, nbands <= 7)
+
+// read-only model and parameters, passed by value to the kernels
+struct DevModel {
+    int nb, nf, nsst, nmax, nstates, nftau;
+    double beta, egs;
+    const int *sizes, *offsets, *connect, *first_state, *state_sst;
+    const long *opoff;
+    const double *ops, *evals, *ftau;   // ftau[f][nftau]
+    const long *dmoff;                  // block-diagonal density-matrix offsets
+    long dmsize;
+    double p_shift, p_glob, p_outer, p_4op;
+    long long nslide;
+    int ntau, nleg, maxhisto, has_szt, meas_dm;
+};
+
+// per-chain scalar state (first member of a chain's memory block)
+struct ChainHdr {
+    int nop, outer, wpos, nwin;
+    int err, cache_pos, kst_sst, mti;
+    double taudiff_max, tauwin_min, tauwin_max, kst_log;
+    double trace_log, trace_sign, det_log, det_sign, cfgsign;
+    double meas_sign;
+    long long cnt_meas;
+    long long try_add, try_rem, try_glob, try_shift;
+    long long acc_add, acc_rem, acc_glob, acc_shift;
+    long long qn_rej, moves;
+    int kf[MAXF];
+};
+
+// byte offsets of the per-chain arrays inside one chain block
+struct ChainLayout {
+    long stride;          // bytes per chain
+    long mt;              // uint32[624]
+    long tau, code;       // double[cap], int[cap]: configuration sorted by tau
+    long fc, fa;          // double[nf][kcap]: creator / annihilator taus per flavour
+    long minv;            // double[nf][kcap*kcap]: inverse hybridisation, column-major
+    long scratch;         // double[kmat + 4*cap]: matrix scratch, then operator scratch
+    long kst;             // double[nmax*nmax]: ket at the window start
+    long bnd;             // double[nbound][nmax*nmax]: bras at window boundaries
+    long bndlog, bndsst;  // double[nbound], int[nbound]
+    long part;            // double[2*nmax]: part traces (log, sign)
+    long kwork;           // double[2][nmax*nmax]: work matrices (large blocks)
+    long ltau, lcode;     // trial operator list (double[lcap], int[lcap])
+    long gtau, gleg, dm, histo, outer; // accumulators
+    int cap, kcap, nbound, lcap;
+    long kmat;            // max(kcap*kcap, nmax*nmax)
+};
+
+}  // namespace w2d

@@ -1,0 +1,314 @@
+// Host-side local Hilbert space (see model.hpp).
+#include "model.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <sstream>
+
+namespace w2d {
+
+// --------------------------------------------------------------------------
+// parameters
+// --------------------------------------------------------------------------
+static std::string trim(const std::string &s)
+{
+    size_t a = s.find_first_not_of(" \t\r");
+    if (a == std::string::npos) return "";
+    size_t b = s.find_last_not_of(" \t\r");
+    return s.substr(a, b - a + 1);
+}
+
+// read_ParameterString (Parameters.F90:281-320): one "key = value" per line, a
+// line starting with '#' is a comment, '#' ends the value, commas become blanks.
+ParamList::ParamList(const std::string &text)
+{
+    std::istringstream in(text);
+    std::string line;
+    while (std::getline(in, line)) {
+        if (!line.empty() && line[0] == '#') continue;
+        for (char &c : line)
+            if (c == '\t') c = ' ';
+        size_t h = line.find('#');
+        if (h != std::string::npos) line = line.substr(0, h);
+        size_t e = line.find('=');
+        if (e == std::string::npos) continue;
+        std::string key = trim(line.substr(0, e));
+        std::string val = trim(line.substr(e + 1));
+        for (char &c : val)
+            if (c == ',') c = ' ';
+        entries_.emplace_back(key, val);
+    }
+}
+
+const std::string *ParamList::find(const std::string &key) const
+{
+    for (const auto &kv : entries_)
+        if (kv.first.compare(0, key.size(), key) == 0) return &kv.second;
+    return nullptr;
+}
+
+bool ParamList::has(const std::string &key) const { return find(key) != nullptr; }
+
+std::string ParamList::str(const std::string &key) const
+{
+    const std::string *v = find(key);
+    return v ? *v : std::string();
+}
+
+double ParamList::real(const std::string &key) const
+{
+    const std::string *v = find(key);
+    if (!v) return -9999999.0;   // RealNotFound
+    std::string s = *v;
+    for (char &c : s)
+        if (c == 'd' || c == 'D') c = 'e';   // Fortran exponent letters
+    return std::strtod(s.c_str(), nullptr);
+}
+
+long long ParamList::integer(const std::string &key) const
+{
+    const std::string *v = find(key);
+    if (!v) return -9999999;     // IntegerNotFound
+    return (long long)std::llround(real(key));
+}
+
+// --------------------------------------------------------------------------
+// symmetric eigenproblem: cyclic Jacobi rotations, eigenvalues ascending
+// --------------------------------------------------------------------------
+static void jacobi_eigh(int n, std::vector<double> &a, std::vector<double> &w, std::vector<double> &v)
+{
+    v.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) v[i + (size_t)i * n] = 1.0;
+    auto A = [&](int i, int j) -> double & { return a[i + (size_t)j * n]; };
+    double norm = 0.0;
+    for (double x : a) norm += x * x;
+    norm = std::sqrt(norm);
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0.0;
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < j; ++i) off += A(i, j) * A(i, j);
+        if (std::sqrt(off) <= 1e-17 * (norm > 0 ? norm : 1.0)) break;
+        for (int p = 0; p < n; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                double apq = A(p, q);
+                if (apq == 0.0) continue;
+                double app = A(p, p), aqq = A(q, q);
+                double theta = (aqq - app) / (2.0 * apq);
+                double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+                double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+                for (int k = 0; k < n; ++k) {
+                    double akp = A(k, p), akq = A(k, q);
+                    A(k, p) = c * akp - s * akq;
+                    A(k, q) = s * akp + c * akq;
+                }
+                for (int k = 0; k < n; ++k) {
+                    double apk = A(p, k), aqk = A(q, k);
+                    A(p, k) = c * apk - s * aqk;
+                    A(q, k) = s * apk + c * aqk;
+                }
+                for (int k = 0; k < n; ++k) {
+                    double vkp = v[k + (size_t)p * n], vkq = v[k + (size_t)q * n];
+                    v[k + (size_t)p * n] = c * vkp - s * vkq;
+                    v[k + (size_t)q * n] = s * vkp + c * vkq;
+                }
+            }
+    }
+    std::vector<int> idx(n);
+    for (int i = 0; i < n; ++i) idx[i] = i;
+    std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return A(x, x) < A(y, y); });
+    w.resize(n);
+    std::vector<double> vs((size_t)n * n);
+    for (int j = 0; j < n; ++j) {
+        w[j] = A(idx[j], idx[j]);
+        for (int i = 0; i < n; ++i) vs[i + (size_t)j * n] = v[i + (size_t)idx[j] * n];
+    }
+    v.swap(vs);
+}
+
+// --------------------------------------------------------------------------
+// model
+// --------------------------------------------------------------------------
+static int popc(unsigned x) { return __builtin_popcount(x); }
+
+// apply c+ (dag = true) or c on bit `bit` of `state`; returns -1 if it vanishes,
+// sign = (-1)^(occupied bits below), Operator.F90:160-172
+static int apply_op(int state, int bit, bool dag, double &sign)
+{
+    int occ = (state >> bit) & 1;
+    if (dag == (occ != 0)) return -1;
+    if (popc((unsigned)state & ((1u << bit) - 1u)) & 1) sign = -sign;
+    return state ^ (1 << bit);
+}
+
+std::string build_local_model(int nb, const double *umat, const double *muimp,
+                              const std::string &qns, LocalModel &m)
+{
+    if (nb < 1 || nb > 7) return "nbands must be 1..7";
+    const bool q_nt = qns.find("Nt") != std::string::npos;
+    const bool q_sz = qns.find("Szt") != std::string::npos;
+    const bool q_qz = qns.find("Qzt") != std::string::npos;
+    for (const char *bad : {"Azt", "Lzt", "Jzt", "All"})
+        if (qns.find(bad) != std::string::npos)
+            return std::string("unsupported quantum number ") + bad;
+    const int ns = 1 << (2 * nb);
+    m.nbands = nb;
+    m.nstates = ns;
+    auto nt = [&](int st) { return popc((unsigned)st); };
+    auto sz2 = [&](int st) {   // 2*Szt, bits 0..N-1 count -1/2 (States.F90:224-234)
+        return popc((unsigned)st >> nb) - popc((unsigned)st & ((1u << nb) - 1u));
+    };
+    auto qz = [&](int st) {    // get_Qzt (States.F90:237-247)
+        double q = 0;
+        for (int b = 0; b < nb; ++b) {
+            int u = (st >> b) & 1, d = (st >> (b + nb)) & 1;
+            if (u != d) q += std::pow(10.0, b);
+        }
+        return q;
+    };
+    // qns2substates (States.F90:151-184)
+    std::vector<int> s2s(ns, -1);
+    int nsst = 0;
+    for (int a = 0; a < ns; ++a) {
+        if (s2s[a] != -1) continue;
+        s2s[a] = nsst;
+        for (int b = a; b < ns; ++b) {
+            if (s2s[b] != -1) continue;
+            if (q_nt && nt(a) != nt(b)) continue;
+            if (q_sz && sz2(a) != sz2(b)) continue;
+            if (q_qz && std::fabs(qz(a) - qz(b)) >= 1e-12) continue;
+            s2s[b] = nsst;
+        }
+        ++nsst;
+    }
+    m.nsst = nsst;
+    m.state_sst = s2s;
+    m.sizes.assign(nsst, 0);
+    m.offsets.assign(nsst, 0);
+    std::vector<std::vector<int>> blk(nsst);
+    for (int st = 0; st < ns; ++st) blk[s2s[st]].push_back(st);
+    std::vector<int> index_in_block(ns);
+    m.states.clear();
+    int off = 0;
+    m.nmax = 0;
+    for (int s = 0; s < nsst; ++s) {
+        m.sizes[s] = (int)blk[s].size();
+        m.offsets[s] = off;
+        off += m.sizes[s];
+        m.nmax = std::max(m.nmax, m.sizes[s]);
+        for (size_t i = 0; i < blk[s].size(); ++i) {
+            index_in_block[blk[s][i]] = (int)i;
+            m.states.push_back(blk[s][i]);
+        }
+    }
+    // Connect table (Operator.F90:140-190)
+    m.connect.assign((size_t)nsst * nb * 4, -1);
+    auto cidx = [&](int s, int b, int sp, int ca0) { return ((s * nb + b) * 2 + sp) * 2 + ca0; };
+    for (int s = 0; s < nsst; ++s)
+        for (int b = 0; b < nb; ++b)
+            for (int sp = 0; sp < 2; ++sp)
+                for (int ca0 = 0; ca0 < 2; ++ca0) {
+                    int target = -1;
+                    for (int st : blk[s]) {
+                        double sg = 1.0;
+                        int t = apply_op(st, b + nb * sp, ca0 == 0, sg);
+                        if (t < 0) continue;
+                        if (target >= 0 && s2s[t] != target) return "operator maps a block onto several blocks";
+                        target = s2s[t];
+                    }
+                    m.connect[cidx(s, b, sp, ca0)] = target;
+                }
+    // H_loc per block: one-particle term plus 1/2 U_1234 c+_1 c+_2 c_4 c_3
+    std::vector<std::vector<double>> h(nsst);
+    for (int s = 0; s < nsst; ++s) h[s].assign((size_t)m.sizes[s] * m.sizes[s], 0.0);
+    auto add_term = [&](double val, const int *bits, const bool *dag, int nops) {
+        for (int s = 0; s < nsst; ++s)
+            for (int st : blk[s]) {
+                int cur = st;
+                double sg = 1.0;
+                for (int o = nops - 1; o >= 0 && cur >= 0; --o) cur = apply_op(cur, bits[o], dag[o], sg);
+                if (cur < 0 || s2s[cur] != s) continue;
+                int n = m.sizes[s];
+                h[s][index_in_block[cur] + (size_t)index_in_block[st] * n] += val * sg;
+            }
+    };
+    const int n2 = 2 * nb;
+    for (int b1 = 0; b1 < nb; ++b1)
+        for (int s1 = 0; s1 < 2; ++s1)
+            for (int b2 = 0; b2 < nb; ++b2)
+                for (int s2 = 0; s2 < 2; ++s2) {
+                    double v = muimp[b1 + nb * s1 + 2 * nb * b2 + 2 * nb * nb * s2];
+                    if (v == 0.0) continue;
+                    int bits[2] = {b1 + nb * s1, b2 + nb * s2};
+                    bool dag[2] = {true, false};
+                    add_term(v, bits, dag, 2);
+                }
+    for (int i = 0; i < n2; ++i)
+        for (int j = 0; j < n2; ++j)
+            for (int k = 0; k < n2; ++k)
+                for (int l = 0; l < n2; ++l) {
+                    double u = umat[i + (size_t)n2 * (j + (size_t)n2 * (k + (size_t)n2 * l))];
+                    if (u == 0.0) continue;
+                    // flavour index 2*b + s (CTQMC.F90 u_matrix layout)
+                    int b[4] = {i / 2, j / 2, k / 2, l / 2}, sp[4] = {i % 2, j % 2, k % 2, l % 2};
+                    int bits[4] = {b[0] + nb * sp[0], b[1] + nb * sp[1], b[3] + nb * sp[3], b[2] + nb * sp[2]};
+                    bool dag[4] = {true, true, false, false};
+                    add_term(0.5 * u, bits, dag, 4);
+                }
+    // force_hermitian, diagonalise, eigenbasis
+    m.evals.assign(ns, 0.0);
+    m.evoff.assign(nsst, 0);
+    m.evecs.clear();
+    std::vector<std::vector<double>> V(nsst);
+    for (int s = 0; s < nsst; ++s) {
+        int n = m.sizes[s];
+        std::vector<double> &a = h[s];
+        for (int i = 0; i < n; ++i)
+            for (int j = i + 1; j < n; ++j) {
+                double x = 0.5 * (a[i + (size_t)j * n] + a[j + (size_t)i * n]);
+                a[i + (size_t)j * n] = a[j + (size_t)i * n] = x;
+            }
+        std::vector<double> w;
+        jacobi_eigh(n, a, w, V[s]);
+        for (int i = 0; i < n; ++i) m.evals[m.offsets[s] + i] = w[i];
+        m.evoff[s] = (long)m.evecs.size();
+        m.evecs.insert(m.evecs.end(), V[s].begin(), V[s].end());
+    }
+    m.egs = m.evals[0];
+    for (double e : m.evals) m.egs = std::min(m.egs, e);
+    // operator blocks in the eigenbasis: V_t^T Psi V_s (Operator.F90:1130-1168)
+    m.opoff.assign(m.connect.size(), -1);
+    m.ops.clear();
+    for (int s = 0; s < nsst; ++s)
+        for (int b = 0; b < nb; ++b)
+            for (int sp = 0; sp < 2; ++sp)
+                for (int ca0 = 0; ca0 < 2; ++ca0) {
+                    int t = m.connect[cidx(s, b, sp, ca0)];
+                    if (t < 0) continue;
+                    int nsrc = m.sizes[s], ntg = m.sizes[t];
+                    std::vector<double> psi((size_t)ntg * nsrc, 0.0);
+                    for (int i = 0; i < nsrc; ++i) {
+                        double sg = 1.0;
+                        int r = apply_op(blk[s][i], b + nb * sp, ca0 == 0, sg);
+                        if (r >= 0) psi[index_in_block[r] + (size_t)i * ntg] = sg;
+                    }
+                    std::vector<double> tmp((size_t)ntg * nsrc, 0.0), out((size_t)ntg * nsrc, 0.0);
+                    for (int j = 0; j < nsrc; ++j)       // tmp = psi * V_s
+                        for (int k = 0; k < nsrc; ++k) {
+                            double v = V[s][k + (size_t)j * nsrc];
+                            if (v == 0.0) continue;
+                            for (int i = 0; i < ntg; ++i) tmp[i + (size_t)j * ntg] += psi[i + (size_t)k * ntg] * v;
+                        }
+                    for (int j = 0; j < nsrc; ++j)       // out = V_t^T * tmp
+                        for (int i = 0; i < ntg; ++i) {
+                            double acc = 0.0;
+                            for (int k = 0; k < ntg; ++k) acc += V[t][k + (size_t)i * ntg] * tmp[k + (size_t)j * ntg];
+                            out[i + (size_t)j * ntg] = acc;
+                        }
+                    m.opoff[cidx(s, b, sp, ca0)] = (long)m.ops.size();
+                    m.ops.insert(m.ops.end(), out.begin(), out.end());
+                }
+    return "";
+}
+
+}  // namespace w2d

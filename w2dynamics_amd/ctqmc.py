@@ -1,0 +1,280 @@
+"""Python host of the MI355X CT-HYB sampling core.
+
+``CtqmcGpu`` drives one handle of the C-ABI (``include/w2dyn_ctqmc.h``), i.e. a
+batch of independent Markov chains on one GPU.  Its methods mirror the entry
+points of the reference solver module ``MCTQMC`` that the Python driver calls
+(``w2dyn/dmft/cthyb_solver.py:240-355``):
+
+=======================  ==================================================
+reference (CTQMC.F90)    here
+=======================  ==================================================
+init_paras + init_rnd    ``CtqmcGpu(params, ..., seed=NSeed)``
++ init_solver            (chain c is seeded NSeed + c)
+set_taudiffmax           ``set_taudiffmax``
+ctqmc_warmup             ``warmup``
+ctqmc_measure(1, 1)      ``measure``
+get/set_mc_config        ``get_config`` / ``set_config``
+module arrays            ``results()`` (per chain) and ``aggregate()``
+dest_ctqmc               ``close``
+=======================  ==================================================
+
+The normalisation applied in ``results`` is the one of ``CTQMC.F90:8663-8771``;
+``aggregate`` combines chains like ``DistributedSample`` combines MPI ranks
+(``w2dyn/auxiliaries/statistics.py:130-209``).
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBNAME = os.path.join(HERE, "libw2dyn_ctqmc.so")
+
+W2D_ERRORS = {1: "invalid argument or unsupported parameter", 2: "HIP runtime error",
+              3: "capacity exceeded (expansion order or window length)",
+              4: "DetRat too small in removal", 5: "zero trace initial state",
+              6: "call out of order"}
+
+
+class CtqmcError(RuntimeError):
+    def __init__(self, code: int, where: str):
+        super().__init__("%s failed: %s (code %d)" % (where, W2D_ERRORS.get(code, "?"), code))
+        self.code = code
+
+
+MOVE_DT = np.dtype([("kind", "<i4"), ("outcome", "<i4"), ("noper", "<i4"), ("outer_sst", "<i4"),
+                    ("window", "<i4"), ("pad", "<i4"), ("trace_log", "<f8"), ("trace_sign", "<f8"),
+                    ("det_log", "<f8"), ("det_sign", "<f8"), ("cfgsign", "<f8")])
+
+_lib = None
+
+
+def load_library(path: str | None = None):
+    """Load the HIP engine; raises if it has not been built (no CPU fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = path or LIBNAME
+    if not os.path.exists(path):
+        raise ImportError("libw2dyn_ctqmc.so not built: run __graft_entry__.build() "
+                          "(expected at %s)" % path)
+    lib = C.CDLL(path)
+    P, I, D = C.c_void_p, C.c_int, C.c_double
+    lib.w2d_ctqmc_create.argtypes = [C.c_char_p, I, I, P, P, P, I, C.c_uint32, I, I, C.POINTER(P)]
+    lib.w2d_ctqmc_set_taudiffmax.argtypes = [P, D]
+    lib.w2d_ctqmc_warmup.argtypes = [P, C.c_int64]
+    lib.w2d_ctqmc_measure.argtypes = [P, C.c_int64]
+    lib.w2d_ctqmc_dims.argtypes = [P] + [C.POINTER(I)] * 7
+    lib.w2d_ctqmc_get_accumulators.argtypes = [P] * 7
+    lib.w2d_ctqmc_get_model.argtypes = [P] * 7
+    lib.w2d_ctqmc_moves_log.argtypes = [P, I, C.c_int64, P]
+    lib.w2d_ctqmc_get_config.argtypes = [P, I, I, C.POINTER(I), P, P, P, P, C.POINTER(I), P]
+    lib.w2d_ctqmc_set_config.argtypes = [P, I, I, P, P, P, P, I, P]
+    lib.w2d_ctqmc_get_minv.argtypes = [P, I, I, C.POINTER(I), P]
+    lib.w2d_ctqmc_last_kernel_ms.argtypes = [P, C.POINTER(D)]
+    lib.w2d_ctqmc_destroy.argtypes = [P]
+    lib.w2d_ctqmc_version.restype = C.c_char_p
+    _lib = lib
+    return lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _check(code: int, where: str):
+    if code != 0:
+        raise CtqmcError(code, where)
+
+
+class CtqmcGpu:
+    """A batch of ``nchains`` CT-HYB Markov chains on one GPU."""
+
+    def __init__(self, params: str, umatrix, ftau, muimp, nchains: int = 1, seed: int = 43890,
+                 capacity: int = 256, device: int = 0):
+        lib = load_library()
+        umatrix = np.asarray(umatrix, dtype=np.float64)
+        nb = int(np.asarray(muimp).shape[0])
+        n2 = 2 * nb
+        u = np.asfortranarray(umatrix.reshape(n2, n2, n2, n2))
+        f = np.asfortranarray(np.asarray(ftau, dtype=np.float64))
+        m = np.asfortranarray(np.asarray(muimp, dtype=np.float64))
+        self._inputs = (u, f, m)
+        h = C.c_void_p()
+        _check(lib.w2d_ctqmc_create(params.encode(), nb, f.shape[-1], _p(u), _p(f), _p(m),
+                                    int(nchains), int(seed) & 0xffffffff, int(capacity), int(device),
+                                    C.byref(h)), "w2d_ctqmc_create")
+        self.h = h
+        self.lib = lib
+        dims = [C.c_int() for _ in range(7)]
+        _check(lib.w2d_ctqmc_dims(h, *[C.byref(x) for x in dims]), "w2d_ctqmc_dims")
+        (self.nbands, self.nchains, self.ntau, self.nleg, self.nsst, self.nstates,
+         self.maxhisto) = [x.value for x in dims]
+        self.beta = _param_value(params, "beta")
+        self._model = None
+
+    @classmethod
+    def from_problem(cls, prob, nchains: int = 1, seed: int = 43890, capacity: int = 256,
+                     device: int = 0):
+        return cls(prob.param_string(), prob.umatrix, prob.ftau, prob.muimp, nchains, seed,
+                   capacity, device)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.w2d_ctqmc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- sampling ---------------------------------------------------------------
+    def set_taudiffmax(self, x: float):
+        _check(self.lib.w2d_ctqmc_set_taudiffmax(self.h, float(x)), "w2d_ctqmc_set_taudiffmax")
+
+    def warmup(self, n: int):
+        _check(self.lib.w2d_ctqmc_warmup(self.h, int(n)), "w2d_ctqmc_warmup")
+
+    def measure(self, nmeas: int):
+        _check(self.lib.w2d_ctqmc_measure(self.h, int(nmeas)), "w2d_ctqmc_measure")
+
+    def moves_log(self, chain: int, n: int) -> np.ndarray:
+        rec = np.zeros(n, dtype=MOVE_DT)
+        _check(self.lib.w2d_ctqmc_moves_log(self.h, int(chain), int(n), _p(rec)),
+               "w2d_ctqmc_moves_log")
+        return rec
+
+    def last_kernel_ms(self) -> float:
+        ms = C.c_double()
+        _check(self.lib.w2d_ctqmc_last_kernel_ms(self.h, C.byref(ms)), "last_kernel_ms")
+        return ms.value
+
+    # -- configurations -----------------------------------------------------------
+    def get_config(self, chain: int = 0) -> dict:
+        n = C.c_int()
+        outer = C.c_int()
+        w = np.zeros(5)
+        _check(self.lib.w2d_ctqmc_get_config(self.h, chain, 0, C.byref(n), None, None, None, None,
+                                             C.byref(outer), _p(w)), "get_config")
+        k = n.value
+        tau = np.zeros(max(k, 1))
+        orb, spin, ca = (np.zeros(max(k, 1), np.int32) for _ in range(3))
+        _check(self.lib.w2d_ctqmc_get_config(self.h, chain, k, C.byref(n), _p(tau), _p(orb), _p(spin),
+                                             _p(ca), C.byref(outer), _p(w)), "get_config")
+        return dict(n=k, taus=tau[:k], orbs=orb[:k], spins=spin[:k], cas=ca[:k],
+                    outer_sst=outer.value, trace_log=w[0], trace_sign=w[1], det_log=w[2],
+                    det_sign=w[3], cfgsign=w[4])
+
+    def set_config(self, chain: int, taus, orbs, spins, cas, outer_sst: int) -> dict:
+        taus = np.ascontiguousarray(taus, np.float64)
+        orbs, spins, cas = (np.ascontiguousarray(x, np.int32) for x in (orbs, spins, cas))
+        w = np.zeros(5)
+        _check(self.lib.w2d_ctqmc_set_config(self.h, chain, len(taus), _p(taus), _p(orbs), _p(spins),
+                                             _p(cas), int(outer_sst), _p(w)), "set_config")
+        return dict(trace_log=w[0], trace_sign=w[1], det_log=w[2], det_sign=w[3], cfgsign=w[4])
+
+    def get_minv(self, chain: int, f: int) -> np.ndarray:
+        k = C.c_int()
+        _check(self.lib.w2d_ctqmc_get_minv(self.h, chain, f, C.byref(k), None), "get_minv")
+        out = np.zeros(max(k.value ** 2, 1))
+        _check(self.lib.w2d_ctqmc_get_minv(self.h, chain, f, C.byref(k), _p(out)), "get_minv")
+        return out[:k.value ** 2].reshape(k.value, k.value, order="F")
+
+    # -- model and results ------------------------------------------------------------
+    def model(self) -> dict:
+        if self._model is None:
+            sizes = np.zeros(self.nsst, np.int32)
+            offs = np.zeros(self.nsst, np.int32)
+            ev = np.zeros(self.nstates)
+            st = np.zeros(self.nstates, np.int32)
+            egs = np.zeros(1)
+            _check(self.lib.w2d_ctqmc_get_model(self.h, _p(sizes), _p(offs), _p(ev), _p(st), None,
+                                                _p(egs)), "get_model")
+            vec = np.zeros(int((sizes.astype(np.int64) ** 2).sum()))
+            _check(self.lib.w2d_ctqmc_get_model(self.h, None, None, None, None, _p(vec), None),
+                   "get_model")
+            evecs, o = [], 0
+            for n in sizes:
+                evecs.append(vec[o:o + n * n].reshape(n, n, order="F"))
+                o += n * n
+            self._model = dict(sizes=sizes, offsets=offs, evals=ev, states=st, evecs=evecs,
+                               egs=float(egs[0]))
+        return self._model
+
+    def accumulators(self) -> dict:
+        nc, nf = self.nchains, 2 * self.nbands
+        g = np.zeros((nc, nf, self.ntau))
+        gl = np.zeros((nc, nf, self.nleg))
+        dm = np.zeros((nc, self.nstates, self.nstates))
+        hi = np.zeros((nc, nf, self.maxhisto + 1))
+        oh = np.zeros((nc, self.nsst))
+        sc = np.zeros((nc, 12))
+        _check(self.lib.w2d_ctqmc_get_accumulators(self.h, _p(g), _p(gl), _p(dm), _p(hi), _p(oh),
+                                                   _p(sc)), "get_accumulators")
+        # dm rows are column-major per chain
+        dm = np.transpose(dm, (0, 2, 1))
+        return dict(gtau=g, gleg=gl, dm=dm, histo=hi, outer_hist=oh, scalars=sc)
+
+    def results(self, acc: dict | None = None) -> dict:
+        """Per-chain observables normalised as in ctqmc_measure; leading axis = chain."""
+        if acc is None:
+            acc = self.accumulators()
+        return normalise(acc, self.nbands, self.beta, self.ntau, self.nleg, self.model())
+
+    def aggregate(self, res: dict | None = None) -> dict:
+        """Mean and standard error over chains (DistributedSample semantics)."""
+        if res is None:
+            res = self.results()
+        out = {}
+        for k, v in res.items():
+            v = np.asarray(v, dtype=float)
+            n = v.shape[0]
+            mean = v.mean(0)
+            err = np.sqrt(((v - mean) ** 2).sum(0) / (n * (n - 1))) if n > 1 else np.full_like(mean, np.nan)
+            out[k] = mean
+            out[k + "_err"] = err
+        return out
+
+
+def _param_value(params: str, key: str) -> float:
+    for line in params.split("\n"):
+        if "=" in line and line.split("=", 1)[0].strip().startswith(key):
+            return float(line.split("=", 1)[1].split("#")[0].strip())
+    raise KeyError(key)
+
+
+def normalise(acc: dict, nb: int, beta: float, ntau: int, nleg: int, model: dict) -> dict:
+    """Normalisation of CTQMC.F90:8663-8771 for every chain of ``acc``."""
+    sc = acc["scalars"]
+    cnt = sc[:, 1]
+    mean_sign = sc[:, 0] / cnt
+    zs = -1.0 / (cnt * mean_sign)
+    nc = sc.shape[0]
+
+    def fl2(a):   # [chain, f, ...] -> [chain, b, s, ...] with f = b + nb*s
+        return a.reshape((nc, 2, nb) + a.shape[2:]).swapaxes(1, 2)
+
+    gtau = fl2(acc["gtau"]) / cnt[:, None, None, None]
+    gtau = -gtau / (-mean_sign)[:, None, None, None] * ntau / beta ** 2
+    lfac = np.sqrt(2.0 * np.arange(nleg) + 1.0) / beta
+    gleg = -fl2(acc["gleg"]) * lfac * zs[:, None, None, None]
+    dm_eb = acc["dm"] * zs[:, None, None]
+    dm_occ = dm_eb.copy()
+    for s, (o, n) in enumerate(zip(model["offsets"], model["sizes"])):
+        v = model["evecs"][s]
+        dm_occ[:, o:o + n, o:o + n] = v @ dm_eb[:, o:o + n, o:o + n] @ v.T
+    diag = np.diagonal(dm_occ, axis1=1, axis2=2)
+    states = model["states"]
+    nocc = np.array([[(states >> (b + nb * s)) & 1 for s in range(2)] for b in range(nb)])
+    occ = np.einsum("ck,ijk,lmk->cijlm", diag, nocc, nocc)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        accr = np.stack([sc[:, 2] / sc[:, 3], sc[:, 4] / sc[:, 5], sc[:, 6] / sc[:, 7],
+                         sc[:, 8] / sc[:, 9]], axis=1)
+    return dict(mean_sign=mean_sign, gtau=gtau, gleg=gleg, occ=occ, densitymatrix=dm_occ,
+                histo=fl2(acc["histo"]) / cnt[:, None, None, None], acc=accr,
+                outer_sst_histo=acc["outer_hist"] / cnt[:, None])
