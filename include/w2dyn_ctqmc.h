@@ -84,8 +84,8 @@ int w2d_ctqmc_dims(const w2d_ctqmc *h, int *nbands, int *nchains, int *ntau, int
  *          block-diagonal, column-major within a chain
  *   histo  [nchains][2N][maxhisto+1]
  *   outer  [nchains][nsst]
- *   scalars[nchains][12]: sum cfgsign, measurements, acc/try add, rem, glob, shift,
- *          attempted moves, qn-rejected moves
+ *   scalars[nchains][14]: sum cfgsign, measurements, acc/try add, rem, glob, shift,
+ *          attempted moves, qn-rejected moves, algorithmic flops, algorithmic bytes
  * flavour index f = b + N*s (orbital fastest). */
 int w2d_ctqmc_get_accumulators(w2d_ctqmc *h, double *gtau, double *gleg, double *dm,
                                double *histo, double *outer, double *scalars);
@@ -119,8 +119,8 @@ int w2d_ctqmc_set_config(w2d_ctqmc *h, int chain, int n, const double *tau, cons
 int w2d_ctqmc_get_minv(w2d_ctqmc *h, int chain, int f, int *k, double *out);
 
 /* Kernel time of the last warmup/measure call in milliseconds (HIP events on
- * the handle's stream). */
-int w2d_ctqmc_last_kernel_ms(const w2d_ctqmc *h, double *ms);
+ * the handle's stream around its launches) and the number of launches. */
+int w2d_ctqmc_last_kernel_ms(const w2d_ctqmc *h, double *ms, int64_t *launches);
 
 /* Replaces dest_ctqmc (CTQMC.F90:901). */
 int w2d_ctqmc_destroy(w2d_ctqmc *h);

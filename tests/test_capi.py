@@ -1,0 +1,109 @@
+"""CPU tests of the C-ABI library and the host-side logic (no GPU compute)."""
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+from w2dynamics_amd import ctqmc
+from w2dynamics_amd import problems as P
+
+HEADER = os.path.join(ROOT, "include", "w2dyn_ctqmc.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(w2d_ctqmc_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctqmc.load_library()
+    names = declared_functions()
+    assert len(names) >= 12
+    for n in names:
+        assert hasattr(lib, n), "missing export " + n
+    assert b"gfx950" in lib.w2d_ctqmc_version()
+
+
+def test_library_missing_fails_loudly(tmp_path):
+    with pytest.raises(ImportError):
+        ctqmc._lib = None
+        try:
+            ctqmc.load_library(str(tmp_path / "nope.so"))
+        finally:
+            ctqmc._lib = None
+    ctqmc.load_library()
+
+
+@pytest.mark.parametrize("override", [{"offdiag": 1}, {"segment": 1}, {"statesampling": 1},
+                                      {"Percentage4OperatorMove": 0.1},
+                                      {"PercentageOuterMove": 0.1},
+                                      {"PercentageWormInsert": 0.2}])
+def test_unsupported_branches_are_rejected(override):
+    """Parameter checks happen before any HIP call and return W2D_ERR_ARG."""
+    p = P.anderson_1orb(**override)
+    with pytest.raises(ctqmc.CtqmcError) as e:
+        ctqmc.CtqmcGpu.from_problem(p)
+    assert e.value.code == 1
+
+
+def test_null_arguments_are_rejected():
+    lib = ctqmc.load_library()
+    h = C.c_void_p()
+    assert lib.w2d_ctqmc_create(None, 1, 10, None, None, None, 1, 0, 0, 0, C.byref(h)) == 1
+    assert lib.w2d_ctqmc_warmup(None, 10) == 1
+    assert lib.w2d_ctqmc_destroy(None) == 0
+
+
+def test_param_string_roundtrip():
+    p = P.kanamori_3orb(Nmeas=123)
+    s = p.param_string()
+    assert ctqmc._param_value(s, "beta") == p.beta
+    assert "QuantumNumbers = Nt Szt #" in s
+    assert ctqmc._param_value(s, "Nmeas") == 123
+
+
+def test_normalise_matches_reference_formulas():
+    """CTQMC.F90:8663-8771 on synthetic accumulators of two chains."""
+    nb, ntau, nleg, beta = 1, 4, 3, 2.0
+    acc = dict(gtau=np.arange(2 * 2 * ntau, dtype=float).reshape(2, 2, ntau),
+               gleg=np.ones((2, 2, nleg)), dm=np.zeros((2, 4, 4)),
+               histo=np.zeros((2, 2, 6)), outer_hist=np.ones((2, 4)),
+               scalars=np.array([[-10.0, 10.0] + [1.0] * 12, [-5.0, 10.0] + [1.0] * 12]))
+    model = dict(offsets=np.arange(4), sizes=np.ones(4, int), evecs=[np.eye(1)] * 4,
+                 states=np.array([0, 1, 2, 3]))
+    r = ctqmc.normalise(acc, nb, beta, ntau, nleg, model)
+    assert np.allclose(r["mean_sign"], [-1.0, -0.5])
+    g0 = acc["gtau"][0].reshape(2, 1, ntau).swapaxes(0, 1) / 10.0 / -1.0 * ntau / beta ** 2
+    assert np.allclose(r["gtau"][0], g0)
+    zs = -1.0 / (10.0 * -0.5)
+    gl = -np.ones((1, 2, nleg)) * np.sqrt(2 * np.arange(nleg) + 1) / beta * zs
+    assert np.allclose(r["gleg"][1], gl)
+
+
+def test_coulomb_u_reduces_to_density_for_f0_only():
+    """With only F0 the Coulomb U is U_{abab} = F0 for all orbital pairs."""
+    u = P.u_coulomb(5, 4.0)
+    for a in range(5):
+        for b in range(5):
+            assert abs(u[a, 0, b, 1, a, 0, b, 1] - 4.0) < 1e-12
+    assert abs(u[0, 0, 1, 1, 1, 0, 0, 1]) < 1e-12   # no exchange without F2/F4
+
+
+def test_coulomb_u_hermitian_symmetries():
+    u = P.u_coulomb(5, *P.slater_from_uj_d(4.0, 0.8))
+    n = 10
+    uf = u.reshape(n, n, n, n)
+    assert np.allclose(uf, uf.transpose(1, 0, 3, 2))          # particle exchange
+    assert np.allclose(uf, uf.transpose(2, 3, 0, 1))          # hermiticity (real U)
+
+
+def test_semicircle_ftau_sum_rule():
+    """F(0) + F(beta) = V^2 = D^2/4 and F is positive and symmetric at half filling."""
+    f = P.semicircle_ftau(1, 10.0, 2.0, 501)[0, 0, 0, 0]
+    assert abs(f[0] + f[-1] - 1.0) < 1e-10
+    assert np.allclose(f, f[::-1], atol=1e-12)
+    assert (f > 0).all()

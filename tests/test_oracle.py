@@ -1,0 +1,293 @@
+"""CPU tests of the oracle: pinned against the reference's golden vectors
+(tests/golden/*.npz, written by tests/golden/make_golden.py from the reference
+core built from its own sources) and against closed forms.
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+from conftest import load_golden, qmc_of
+from oracle import OracleChain, OracleModel, rng_stream
+from w2dynamics_amd import problems as P
+
+LOG_CASES = ["c1_log_w1", "c1_log_win", "c2_log_win", "c3_log_win", "c4_log_w1"]
+FIELDS = ("kind", "outcome", "noper", "outer_sst", "window")
+
+
+# ---------------------------------------------------------------- RNG -------
+def test_mt19937_known_answer():
+    """C++ [rand.predef]: the 10000th output of a default-constructed mt19937
+    (seed 5489) is 4123659995."""
+    out = rng_stream(5489, np.full(10000, 2))
+    assert int(out[-1]) == 4123659995
+
+
+def test_grnd_is_53bit_canonical():
+    """generate_canonical<double,53> from two 32-bit draws: (g1 + g2 2^32) / 2^64."""
+    raw = rng_stream(42, np.full(2000, 2))
+    d = rng_stream(42, np.zeros(1000, np.int32))
+    g1, g2 = raw[0::2], raw[1::2]
+    expect = (g1 + g2 * 2.0 ** 32) / 2.0 ** 64
+    assert np.array_equal(d, expect)
+    assert (d >= 0).all() and (d < 1).all()
+
+
+def test_randint_lemire_range_and_uniformity():
+    n = 20000
+    lo = np.full(n, 1, np.int32)
+    hi = np.full(n, 6, np.int32)
+    v = rng_stream(7, np.ones(n, np.int32), lo, hi)
+    assert v.min() == 1 and v.max() == 6
+    counts = np.bincount(v.astype(int))[1:]
+    assert np.all(np.abs(counts - n / 6) < 5 * math.sqrt(n / 6))
+    # range 1 consumes exactly one draw and returns the lower bound
+    w = rng_stream(7, np.array([1, 2]), np.array([3, 0]), np.array([3, 0]))
+    assert w[0] == 3 and w[1] == rng_stream(7, np.array([2, 2]))[1]
+
+
+# ------------------------------------------------------------- model --------
+@pytest.mark.parametrize("name", ["c1_log_w1", "c2_log_win", "c3_log_win", "c4_log_w1"])
+def test_model_matches_reference(name):
+    z = load_golden(name)
+    m = OracleModel(z["umatrix"], z["muimp"])
+    assert m.nsst == int(z["model_nsst"])
+    assert np.array_equal(m.sizes, z["model_sizes"])
+    assert np.array_equal(np.concatenate(m.states), z["model_states"])
+    assert np.array_equal(m.connect, z["model_connect"])
+    ev = np.concatenate(m.evals)
+    assert np.abs(ev - z["model_evals"]).max() <= 1e-11 * max(1.0, np.abs(ev).max())
+    assert abs(m.egs - float(z["model_egs"])) <= 1e-11 * max(1.0, abs(m.egs))
+
+
+def test_model_spectrum_matches_fock_space_ed():
+    """Block spectra equal the eigenvalues of the full Fock-space Hamiltonian."""
+    p = P.kanamori_3orb(beta=20.0)
+    m = OracleModel(p.umatrix, p.muimp)
+    h = P.fock_hamiltonian(p.umatrix, p.muimp)
+    assert np.allclose(np.sort(np.concatenate(m.evals)), np.linalg.eigvalsh(h), atol=1e-10)
+
+
+def test_eigenbasis_operators_anticommute():
+    """{c_i, c+_j} = delta_ij on every block (eigenbasis blocks)."""
+    p = P.kanamori_2orb(beta=10.0)
+    m = OracleModel(p.umatrix, p.muimp)
+    nb = m.nbands
+    for s in range(m.nsst):
+        n = m.sizes[s]
+        for b1 in range(nb):
+            for s1 in range(2):
+                for b2 in range(nb):
+                    for s2 in range(2):
+                        acc = np.zeros((n, n))
+                        t = m.connect[s, b2, s2, 0]          # c+_2 first
+                        if t >= 0 and m.connect[t, b1, s1, 1] == s:
+                            acc += m.psi_eb[(t, b1, s1, 1)] @ m.psi_eb[(s, b2, s2, 0)]
+                        t = m.connect[s, b1, s1, 1]          # c_1 first
+                        if t >= 0 and m.connect[t, b2, s2, 0] == s:
+                            acc += m.psi_eb[(t, b2, s2, 0)] @ m.psi_eb[(s, b1, s1, 1)]
+                        want = np.eye(n) if (b1, s1) == (b2, s2) else np.zeros((n, n))
+                        assert np.allclose(acc, want, atol=1e-12)
+
+
+# --------------------------------------------------------- move logs --------
+@pytest.mark.parametrize("name", LOG_CASES)
+def test_oracle_move_log_matches_reference(name):
+    z = load_golden(name)
+    q = qmc_of(z)
+    ch = OracleChain(z["umatrix"], z["ftau"], z["muimp"], float(z["beta"]), q, int(z["seed"]))
+    ch.set_taudiffmax(float(q["TaudiffMax"]))
+    rec = ch.moves(int(z["nmoves"]))
+    for f in FIELDS:
+        bad = np.nonzero(rec[f] != z["move_" + f])[0]
+        assert len(bad) == 0, "%s: %s differs first at move %d" % (name, f, bad[0])
+    assert np.abs(rec["trace_log"] - z["move_trace_log"]).max() < 1e-10
+    assert np.abs(rec["det_log"] - z["move_det_log"]).max() < 1e-10
+    for f in ("trace_sign", "det_sign", "cfgsign"):
+        assert np.array_equal(rec[f], z["move_" + f])
+    # final configuration and inverse hybridisation blocks
+    cfg = ch.config()
+    assert cfg["n"] == int(z["cfg_n"])
+    assert np.allclose(cfg["taus"], z["cfg_taus"], rtol=0, atol=0)
+    nf = 2 * int(z["nbands"])
+    for f in range(nf):
+        a, b = ch.minv(f), z["cfg_minv_%d" % f]
+        assert a.shape == b.shape
+        if a.size:
+            assert np.abs(a - b).max() <= 1e-9 * max(1.0, np.abs(b).max())
+
+
+@pytest.mark.parametrize("name", ["c1_run", "c2_run"])
+def test_oracle_observables_match_reference(name):
+    z = load_golden(name)
+    q = qmc_of(z)
+    ch = OracleChain(z["umatrix"], z["ftau"], z["muimp"], float(z["beta"]), q, int(z["seed"]))
+    ch.set_taudiffmax(float(q["TaudiffMax"]))
+    ch.warmup(int(z["nwarm"]))
+    ch.measure(int(q["Nmeas"]), int(q["NCorr"]))
+    r = ch.results()
+    for k in ("mean_sign", "gtau", "gleg", "occ", "densitymatrix", "histo", "acc",
+              "outer_sst_histo"):
+        a, b = np.asarray(r[k]), np.asarray(z["res_" + k])
+        assert a.shape == b.shape, k
+        assert np.allclose(a, b, rtol=1e-10, atol=1e-12), (k, np.abs(a - b).max())
+
+
+# ------------------------------------------------------ closed forms --------
+def test_empty_trace_is_boltzmann_sum():
+    p = P.kanamori_2orb(beta=10.0)
+    ch = OracleChain.from_problem(p, 1)
+    m = ch.model
+    for s in range(m.nsst):
+        w = ch.set_config([], [], [], [], s)
+        expect = np.sum(np.exp(-p.beta * (m.evals[s] - m.egs)))
+        assert w["trace_sign"] == 1.0
+        assert abs(w["trace_log"] - math.log(expect)) < 1e-12
+
+
+def test_two_operator_trace_closed_form():
+    """Tr_s[e^{-(beta-t2) H} c e^{-(t2-t1) H} c+ e^{-t1 H}] from the eigenbasis."""
+    p = P.kanamori_2orb(beta=8.0)
+    ch = OracleChain.from_problem(p, 3)
+    m = ch.model
+    t1, t2 = 1.3, 5.2
+    for s in range(m.nsst):
+        for b in range(2):
+            for sp in range(2):
+                t = m.connect[s, b, sp, 0]
+                if t < 0 or m.connect[t, b, sp, 1] != s:
+                    continue
+                cd = m.psi_eb[(s, b, sp, 0)]
+                c = m.psi_eb[(t, b, sp, 1)]
+                es, et = m.evals[s] - m.egs, m.evals[t] - m.egs
+                mat = (np.diag(np.exp(-(p.beta - t2) * es)) @ c @ np.diag(np.exp(-(t2 - t1) * et))
+                       @ cd @ np.diag(np.exp(-t1 * es)))
+                val = np.trace(mat)
+                w = ch.set_config([t1, t2], [b, b], [sp, sp], [1, 2], s)
+                assert abs(w["trace_sign"] * math.exp(w["trace_log"]) - val) <= 1e-12 * abs(val)
+
+
+def test_determinant_and_inverse_brute_force():
+    """det and M = Delta^{-1} of a visited configuration against numpy."""
+    p = P.kanamori_2orb(beta=20.0, TaudiffMax=4.0)
+    ch = OracleChain.from_problem(p, 5)
+    ch.set_taudiffmax(4.0)
+    ch.moves(4000, record=False)
+    cfg = ch.config()
+    nb = 2
+    beta = p.beta
+    f_of = lambda b, s: b + nb * s  # noqa: E731
+    ftab = p.ftau
+    nft = ftab.shape[-1]
+    tot = 0.0
+    sgn = 1.0
+    for f in range(2 * nb):
+        b, s = f % nb, f // nb
+        sel = (cfg["orbs"] == b) & (cfg["spins"] == s)
+        tc = cfg["taus"][sel & (cfg["cas"] == 1)]
+        ta = cfg["taus"][sel & (cfg["cas"] == 2)]
+        if len(tc) == 0:
+            continue
+
+        def F(t):
+            sig = np.where(t < 0, -1.0, 1.0)
+            t = np.where(t < 0, t + beta, t)
+            n = t / beta * (nft - 1)
+            i = n.astype(int)
+            g = ftab[b, s, b, s]
+            return sig * (g[i] + (n - i) * (g[i + 1] - g[i]))
+        D = F(ta[:, None] - tc[None, :])
+        sd, ld = np.linalg.slogdet(D)
+        tot += ld
+        sgn *= sd
+        assert np.allclose(ch.minv(f), np.linalg.inv(D), rtol=1e-8, atol=1e-10)
+        assert np.allclose(ch.minv(f) @ D, np.eye(len(tc)), atol=1e-9)
+    assert abs(cfg["det_log"] - tot) < 1e-9
+    assert cfg["det_sign"] == sgn
+    del f_of
+
+
+def test_legendre_recursion_matches_numpy():
+    """The Bonnet recursion of MeasGtauRem equals numpy's Legendre polynomials."""
+    from numpy.polynomial import legendre as L
+    x = np.linspace(-1, 1, 37)
+    pl = np.zeros_like(x)
+    plm1 = np.zeros_like(x)
+    for l in range(30):
+        if l == 0:
+            plp1 = np.ones_like(x)
+        elif l == 1:
+            plp1 = x.copy()
+        else:
+            ll = l - 1
+            plp1 = (2 * ll + 1) / (ll + 1) * x * pl - ll / (ll + 1) * plm1
+        assert np.allclose(plp1, L.legval(x, np.eye(30)[l]), atol=1e-12)
+        plm1, pl = pl, plp1
+
+
+def _anderson_ed(eps_d, u, eps_k, v_k, beta):
+    """Exact <n_up> and <n_up n_dn> of a single-orbital Anderson impurity with a
+    discrete bath, from the full Fock space (bits: d_up, d_dn, then bath)."""
+    nbath = len(eps_k)
+    nso = 2 + 2 * nbath
+    dim = 1 << nso
+
+    def mode(site, s):   # site 0 = impurity, 1.. = bath
+        return 2 * site + s
+
+    def hop(state, a, b):   # c+_a c_b |state>
+        if not (state >> b) & 1:
+            return None, 0
+        sg = (-1) ** bin(state & ((1 << b) - 1)).count("1")
+        st = state ^ (1 << b)
+        if (st >> a) & 1:
+            return None, 0
+        sg *= (-1) ** bin(st & ((1 << a) - 1)).count("1")
+        return st ^ (1 << a), sg
+
+    h = np.zeros((dim, dim))
+    for st in range(dim):
+        nu, nd = (st >> 0) & 1, (st >> 1) & 1
+        e = eps_d * (nu + nd) + u * nu * nd
+        for k in range(nbath):
+            for s in range(2):
+                e += eps_k[k] * ((st >> mode(k + 1, s)) & 1)
+        h[st, st] = e
+        for k in range(nbath):
+            for s in range(2):
+                for a, b in ((mode(0, s), mode(k + 1, s)), (mode(k + 1, s), mode(0, s))):
+                    t, sg = hop(st, a, b)
+                    if t is not None:
+                        h[t, st] += v_k[k] * sg
+    w, vec = np.linalg.eigh(h)
+    p = np.exp(-beta * (w - w[0]))
+    p /= p.sum()
+    nu = np.array([(st >> 0) & 1 for st in range(dim)], float)
+    nd = np.array([(st >> 1) & 1 for st in range(dim)], float)
+    prob = (vec ** 2) @ p
+    return float(prob @ nu), float(prob @ (nu * nd))
+
+
+def test_full_run_matches_exact_diagonalisation():
+    """Impurity + 3 discrete bath sites: <n_up> and double occupancy of the
+    sampled density matrix agree with exact diagonalisation within 4 sigma
+    (sigma from independent chains)."""
+    beta, u, mu = 5.0, 2.0, 0.6
+    eps_k, v_k = np.array([-1.0, 0.2, 1.1]), np.array([0.5, 0.35, 0.45])
+    p = P.anderson_1orb(beta=beta, u=u, nftau=4001)
+    p.ftau = P.discrete_bath_ftau(beta, eps_k, v_k, 4001)
+    p.muimp = P.solver_muimp(1, mu)
+    n_ed, d_ed = _anderson_ed(-mu, u, eps_k, v_k, beta)
+    nup, docc = [], []
+    for seed in range(8):
+        ch = OracleChain.from_problem(p, 100 + seed)
+        ch.warmup(5000)
+        ch.measure(2000, 20)
+        r = ch.results()
+        nup.append(r["occ"][0, 0, 0, 0])
+        docc.append(r["occ"][0, 0, 0, 1])
+    for vals, exact in ((nup, n_ed), (docc, d_ed)):
+        m = np.mean(vals)
+        se = np.std(vals, ddof=1) / np.sqrt(len(vals))
+        assert abs(m - exact) < 4 * se + 2e-3, (m, se, exact)

@@ -289,6 +289,16 @@ DEV void apply_block(const double *P, const double *Kin, double *Kout, int nt, i
     wsync();
 }
 
+// algorithmic work of one operator application on `no` outer states: a
+// (nt x ns) block times an (ns x no) state matrix plus the diagonal propagator;
+// bytes = operator block + states in and out
+template <int NM>
+DEV void trace_work(Ctx<NM> &c, int nt, int ns, int no)
+{
+    c.h.alg_flops += (double)no * (2.0 * nt * ns + ns);
+    c.h.alg_bytes += 8.0 * nt * ns + 8.0 * no * (ns + nt);
+}
+
 // Propagate a ket matrix upward in tau through `n` listed operators starting at
 // tau t0, ending at tau t1.  *K (in/out) is swapped between K0/K1.
 template <int NM>
@@ -302,6 +312,7 @@ DEV void ket_sweep(Ctx<NM> &c, double *&K, double *&Kw, int &s, double &lg, int 
         int cj = ocode[j];
         prop_rows(c, K, s, no, tj - t, lg);
         int tsst = conn(M, s, cj, false);
+        trace_work(c, M.sizes[tsst], M.sizes[s], no);
         apply_block<NM>(opblock(M, s, cj, false), K, Kw, M.sizes[tsst], M.sizes[s], no);
         double *x = K;
         K = Kw;
@@ -327,6 +338,7 @@ DEV void bra_sweep(Ctx<NM> &c, double *&B, double *&Bw, int &s, double &lg, int 
         int cj = ocode[j];
         prop_rows(c, B, s, no, t - tj, lg);
         int tsst = conn(M, s, cj, true);
+        trace_work(c, M.sizes[tsst], M.sizes[s], no);
         apply_block<NM>(opblock(M, s, cj, true), B, Bw, M.sizes[tsst], M.sizes[s], no);
         double *x = B;
         B = Bw;
@@ -957,6 +969,8 @@ DEV int step_add(Ctx<NM> &c)
     }
     wsync();
     double S = detrat_add(c, c.mi(f), k, hybr(M, ta - tc, f));
+    c.h.alg_flops += 2.0 * k * k;
+    c.h.alg_bytes += 8.0 * k * k;
     double sig = (abs(fp_c - fp_a) % 2) ? -1.0 : 1.0;
     double dr_log = log(fabs(S)), dr_sign = copysign(1.0, S) * sig;
     double tl, ts;
@@ -971,6 +985,8 @@ DEV int step_add(Ctx<NM> &c)
     // accept: M, operator list, flavour lists, weights, sign
     double *colb = c.dvec, *rowb = c.dvec + c.L.kcap + 1;
     mat_add(c, c.mi(f), k, S, fp_c, fp_a, colb, rowb);
+    c.h.alg_flops += 6.0 * k * k;
+    c.h.alg_bytes += 8.0 * k * k + 8.0 * (k + 1) * (k + 1);
     int g1 = ia + q1, g2 = ia + q2 + 1;
     ops_shift(c.tau, c.h.nop, ia + q2, 2);
     ops_shift(c.code, c.h.nop, ia + q2, 2);
@@ -1100,6 +1116,8 @@ DEV int step_rem(Ctx<NM> &c)
         return OUT_REJ;
     }
     mat_rem(c, c.mi(f), k, fp_c, fp_a, c.dvec, c.dvec + c.L.kcap);
+    c.h.alg_flops += 3.0 * k * k;
+    c.h.alg_bytes += 8.0 * k * k + 8.0 * (k - 1) * (k - 1);
     ops_shift(c.tau, i2, i1 + 1, -1);
     ops_shift(c.code, i2, i1 + 1, -1);
     ops_shift(c.tau, c.h.nop, i2 + 1, -2);
@@ -1273,6 +1291,11 @@ DEV int step_glob(Ctx<NM> &c)
             dol += l;
             dos *= s;
         }
+    }
+    for (int f = 0; f < nf; ++f) {
+        double k3 = (double)c.h.kf[f] * c.h.kf[f] * c.h.kf[f], ko3 = (double)kold[f] * kold[f] * kold[f];
+        c.h.alg_flops += (2.0 / 3.0) * (k3 + ko3) * 2.0;
+        c.h.alg_bytes += 8.0 * ((double)c.h.kf[f] * c.h.kf[f] + (double)kold[f] * kold[f]);
     }
     double tl, ts;
     full_trace(c, c.h.outer, tl, ts);
@@ -1488,6 +1511,8 @@ DEV void meas_gtau(Ctx<NM> &c, double *gtau, double *gleg)
         const int k = c.h.kf[f];
         const double *ctl = c.fc + (long)f * c.L.kcap, *atl = c.fa + (long)f * c.L.kcap;
         const double *Mf = c.mi(f);
+        c.h.alg_flops += (double)k * k * (5.0 * M.nleg + 4.0);
+        c.h.alg_bytes += 8.0 * k * k;
         for (int p0 = 0; p0 < k * k; p0 += WAVE) {
             int p = p0 + lane_id();
             bool act = p < k * k;
@@ -1801,6 +1826,7 @@ struct w2d_ctqmc {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
+    long long last_launches = 0;
     std::vector<void *> dev_allocs;
     char *chains = nullptr;
     bool measured = false;
@@ -1996,6 +2022,7 @@ void reset_counters(ChainHdr &x)
     x.try_add = x.try_rem = x.try_glob = x.try_shift = 0;
     x.acc_add = x.acc_rem = x.acc_glob = x.acc_shift = 0;
     x.qn_rej = x.moves = 0;
+    x.alg_flops = x.alg_bytes = 0.0;
 }
 
 int map_err(int e)
@@ -2014,7 +2041,8 @@ int run_moves(w2d_ctqmc *h, long long nmoves, long long ncorr)
 {
     long long per = ncorr > 0 ? ncorr * std::max<long long>(1, 4000 / ncorr) : 4000;
     HIPCHK(hipEventRecord(h->ev0, h->stream));
-    for (long long done = 0; done < nmoves; done += per) {
+    h->last_launches = 0;
+    for (long long done = 0; done < nmoves; done += per, ++h->last_launches) {
         long long n = std::min(per, nmoves - done);
         DISPATCH(launch_sweep, h, n, ncorr, -1, nullptr);
         HIPCHK(hipGetLastError());
@@ -2242,7 +2270,7 @@ int w2d_ctqmc_get_accumulators(w2d_ctqmc *h, double *gtau, double *gleg, double 
             }
         }
         if (scalars) {
-            double *S = scalars + 12L * c;
+            double *S = scalars + 14L * c;
             const ChainHdr &x = hd[c];
             S[0] = x.meas_sign;
             S[1] = (double)x.cnt_meas;
@@ -2251,6 +2279,7 @@ int w2d_ctqmc_get_accumulators(w2d_ctqmc *h, double *gtau, double *gleg, double 
             S[6] = (double)x.acc_glob, S[7] = (double)x.try_glob;
             S[8] = (double)x.acc_shift, S[9] = (double)x.try_shift;
             S[10] = (double)x.moves, S[11] = (double)x.qn_rej;
+            S[12] = x.alg_flops, S[13] = x.alg_bytes;
         }
     }
     return W2D_OK;
@@ -2378,10 +2407,11 @@ int w2d_ctqmc_get_minv(w2d_ctqmc *h, int chain, int f, int *k, double *out)
     return W2D_OK;
 }
 
-int w2d_ctqmc_last_kernel_ms(const w2d_ctqmc *h, double *ms)
+int w2d_ctqmc_last_kernel_ms(const w2d_ctqmc *h, double *ms, int64_t *launches)
 {
     if (!h || !ms) return W2D_ERR_ARG;
     *ms = h->last_ms;
+    if (launches) *launches = h->last_launches;
     return W2D_OK;
 }
 

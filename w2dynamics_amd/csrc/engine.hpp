@@ -33,6 +33,7 @@ struct ChainHdr {
     long long try_add, try_rem, try_glob, try_shift;
     long long acc_add, acc_rem, acc_glob, acc_shift;
     long long qn_rej, moves;
+    double alg_flops, alg_bytes;   // algorithmic work of this chain's moves and measurements
     int kf[MAXF];
 };
 

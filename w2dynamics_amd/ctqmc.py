@@ -74,7 +74,7 @@ def load_library(path: str | None = None):
     lib.w2d_ctqmc_get_config.argtypes = [P, I, I, C.POINTER(I), P, P, P, P, C.POINTER(I), P]
     lib.w2d_ctqmc_set_config.argtypes = [P, I, I, P, P, P, P, I, P]
     lib.w2d_ctqmc_get_minv.argtypes = [P, I, I, C.POINTER(I), P]
-    lib.w2d_ctqmc_last_kernel_ms.argtypes = [P, C.POINTER(D)]
+    lib.w2d_ctqmc_last_kernel_ms.argtypes = [P, C.POINTER(D), C.POINTER(C.c_int64)]
     lib.w2d_ctqmc_destroy.argtypes = [P]
     lib.w2d_ctqmc_version.restype = C.c_char_p
     _lib = lib
@@ -149,10 +149,12 @@ class CtqmcGpu:
                "w2d_ctqmc_moves_log")
         return rec
 
-    def last_kernel_ms(self) -> float:
+    def last_kernel_ms(self):
+        """(milliseconds, launches) of the last warmup/measure call."""
         ms = C.c_double()
-        _check(self.lib.w2d_ctqmc_last_kernel_ms(self.h, C.byref(ms)), "last_kernel_ms")
-        return ms.value
+        nl = C.c_int64()
+        _check(self.lib.w2d_ctqmc_last_kernel_ms(self.h, C.byref(ms), C.byref(nl)), "last_kernel_ms")
+        return ms.value, nl.value
 
     # -- configurations -----------------------------------------------------------
     def get_config(self, chain: int = 0) -> dict:
@@ -213,7 +215,7 @@ class CtqmcGpu:
         dm = np.zeros((nc, self.nstates, self.nstates))
         hi = np.zeros((nc, nf, self.maxhisto + 1))
         oh = np.zeros((nc, self.nsst))
-        sc = np.zeros((nc, 12))
+        sc = np.zeros((nc, 14))
         _check(self.lib.w2d_ctqmc_get_accumulators(self.h, _p(g), _p(gl), _p(dm), _p(hi), _p(oh),
                                                    _p(sc)), "get_accumulators")
         # dm rows are column-major per chain
