@@ -1,0 +1,2151 @@
+// Device side of the batched CT-HYB (matrix/"M-V" Z-space) Monte Carlo engine
+// for MI355X (gfx950).
+//
+// One 64-lane wavefront owns one Markov chain and runs NCorr-long sweeps of
+// Metropolis steps inside one persistent launch.  Control logic (random draws,
+// proposals, window bookkeeping) runs uniformly on all lanes against a chain
+// header held in LDS; the heavy parts are lane-parallel:
+//   * local trace (Trace.F90:5354-6102): block matrix products
+//       K <- Psi_EB(b,s,ca) * diag(exp(-(E-E0) dtau)) * K
+//     over the operators of the sliding window, starting from a cached ket at the
+//     window's lower edge and contracted with a cached bra at its upper edge; K
+//     holds one column per state of the outer superstate, lanes own (row, col)
+//     entries, one log scale per matrix, one barrier per operator;
+//   * determinant ratio S - R^T M Q and the rank-1 grow/shrink of M = Delta^-1
+//     (MatrixUpdate.F90:392-482): lanes own rows of the k x k column-major block;
+//   * G(tau)/Legendre accumulation (CTQMC.F90:2252-2440): lanes own (a, c) pairs.
+// The random-number stream, proposal and acceptance formulas and the window
+// schedule follow the reference exactly, so chain c seeded with seed0 + c makes
+// the reference's accept/reject decisions for init_rnd(seed0 + c).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+
+#include "../../include/w2dyn_ctqmc.h"
+#include "engine.hpp"
+
+namespace w2d {
+
+#define DEV __device__ __forceinline__
+
+// ---------------------------------------------------------------------------
+// wave helpers (one wavefront per block)
+// ---------------------------------------------------------------------------
+DEV int lane_id() { return (int)threadIdx.x; }
+
+// Cross-lane reductions with DPP row operations (VALU, no LDS traffic): quad
+// permutes and row mirrors reduce each 16-lane row, readlane combines the four
+// rows; the result is uniform.
+template <int CTRL>
+DEV int dpp_i(int v)
+{
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xf, 0xf, false);
+}
+
+template <int CTRL>
+DEV double dpp_d(double v)
+{
+    const int lo = __double2loint(v), hi = __double2hiint(v);
+    return __hiloint2double(dpp_i<CTRL>(hi), dpp_i<CTRL>(lo));
+}
+
+DEV double readlane_d(double v, int l)
+{
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                            __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+
+constexpr int DPP_QP_1032 = 0xB1, DPP_QP_2301 = 0x4E, DPP_ROW_HALF_MIRROR = 0x141, DPP_ROW_MIRROR = 0x140;
+
+DEV double wave_max(double v)
+{
+    v = fmax(v, dpp_d<DPP_QP_1032>(v));
+    v = fmax(v, dpp_d<DPP_QP_2301>(v));
+    v = fmax(v, dpp_d<DPP_ROW_HALF_MIRROR>(v));
+    v = fmax(v, dpp_d<DPP_ROW_MIRROR>(v));
+    return fmax(fmax(readlane_d(v, 0), readlane_d(v, 16)), fmax(readlane_d(v, 32), readlane_d(v, 48)));
+}
+
+DEV double wave_sum(double v)
+{
+    v += dpp_d<DPP_QP_1032>(v);
+    v += dpp_d<DPP_QP_2301>(v);
+    v += dpp_d<DPP_ROW_HALF_MIRROR>(v);
+    v += dpp_d<DPP_ROW_MIRROR>(v);
+    return (readlane_d(v, 0) + readlane_d(v, 16)) + (readlane_d(v, 32) + readlane_d(v, 48));
+}
+
+DEV int wave_isum(int v)
+{
+    v += dpp_i<DPP_QP_1032>(v);
+    v += dpp_i<DPP_QP_2301>(v);
+    v += dpp_i<DPP_ROW_HALF_MIRROR>(v);
+    v += dpp_i<DPP_ROW_MIRROR>(v);
+    return (__builtin_amdgcn_readlane(v, 0) + __builtin_amdgcn_readlane(v, 16)) +
+           (__builtin_amdgcn_readlane(v, 32) + __builtin_amdgcn_readlane(v, 48));
+}
+
+DEV int wave_imax(int v)
+{
+    v = max(v, dpp_i<DPP_QP_1032>(v));
+    v = max(v, dpp_i<DPP_QP_2301>(v));
+    v = max(v, dpp_i<DPP_ROW_HALF_MIRROR>(v));
+    v = max(v, dpp_i<DPP_ROW_MIRROR>(v));
+    return max(max(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16)),
+               max(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
+}
+
+DEV int wave_imin(int v) { return -wave_imax(-v); }
+
+DEV int wave_excl_scan(int v, int *total)
+{
+    int x = v;
+    for (int o = 1; o < WAVE; o <<= 1) {
+        int y = __shfl_up(x, o, WAVE);
+        if (lane_id() >= o) x += y;
+    }
+    *total = __shfl(x, WAVE - 1, WAVE);
+    return x - v;
+}
+
+// hardware FP64 add to device (global) memory, result unused
+DEV void global_add(double *p, double v)
+{
+    typedef __attribute__((address_space(1))) double gdouble;
+    (void)__builtin_amdgcn_global_atomic_fadd_f64((gdouble *)p, v);
+}
+
+DEV int popc64(unsigned long long m) { return __popcll(m); }
+
+DEV unsigned long long lanemask_le() { return (lane_id() == 63) ? ~0ull : ((2ull << lane_id()) - 1ull); }
+
+DEV void wsync() { __syncthreads(); }
+
+// ---------------------------------------------------------------------------
+// random numbers: mt19937 + libstdc++ generate_canonical<double,53> and
+// uniform_int_distribution (src/mtrng/random.cc:30-79).  The 624-word state is
+// shared in LDS; every lane draws the same numbers, the twist is lane-parallel.
+// ---------------------------------------------------------------------------
+struct Rng {
+    uint32_t *mt;   // LDS
+    int *idx;       // LDS (uniform)
+};
+
+DEV uint32_t mt_twist_word(uint32_t a, uint32_t b, uint32_t c)
+{
+    uint32_t y = (a & 0x80000000u) | (b & 0x7fffffffu);
+    uint32_t v = c ^ (y >> 1);
+    if (y & 1u) v ^= 0x9908b0dfu;
+    return v;
+}
+
+// new mt[i] depends on mt[i+1] (old) and mt[i+397]: old for i < 227, new for
+// i >= 227 (from index i-227).  Phases [0,227) [227,454) [454,623) {623}.
+DEV void mt_regen(uint32_t *mt)
+{
+    const int l = lane_id();
+    uint32_t nv[4];
+    int c = 0;
+    for (int i = l; i < 227; i += WAVE) nv[c++] = mt_twist_word(mt[i], mt[i + 1], mt[i + 397]);
+    wsync();
+    c = 0;
+    for (int i = l; i < 227; i += WAVE) mt[i] = nv[c++];
+    wsync();
+    c = 0;
+    for (int i = 227 + l; i < 454; i += WAVE) nv[c++] = mt_twist_word(mt[i], mt[i + 1], mt[i - 227]);
+    wsync();
+    c = 0;
+    for (int i = 227 + l; i < 454; i += WAVE) mt[i] = nv[c++];
+    wsync();
+    c = 0;
+    for (int i = 454 + l; i < 623; i += WAVE) nv[c++] = mt_twist_word(mt[i], mt[i + 1], mt[i - 227]);
+    wsync();
+    c = 0;
+    for (int i = 454 + l; i < 623; i += WAVE) mt[i] = nv[c++];
+    wsync();
+    uint32_t last = mt_twist_word(mt[623], mt[0], mt[396]);
+    wsync();
+    if (l == 0) mt[623] = last;
+    wsync();
+}
+
+DEV uint32_t mt_next(Rng &r)
+{
+    int i = *r.idx;
+    if (i >= 624) {
+        mt_regen(r.mt);
+        i = 0;
+    }
+    uint32_t y = r.mt[i];
+    *r.idx = i + 1;   // all lanes store the same value
+    y ^= y >> 11;
+    y ^= (y << 7) & 0x9d2c5680u;
+    y ^= (y << 15) & 0xefc60000u;
+    y ^= y >> 18;
+    return y;
+}
+
+DEV double grnd(Rng &r)
+{
+    double sum = (double)mt_next(r);
+    const double tmp = 4294967296.0;
+    sum = __dadd_rn(sum, __dmul_rn((double)mt_next(r), tmp));
+    double ret = sum / (tmp * tmp);
+    if (ret >= 1.0) ret = 0x1.fffffffffffffp-1;
+    return ret;
+}
+
+DEV int randint(Rng &r, int a, int b)
+{
+    uint32_t range = (uint32_t)b - (uint32_t)a + 1u;
+    uint64_t prod = (uint64_t)mt_next(r) * (uint64_t)range;
+    uint32_t low = (uint32_t)prod;
+    if (low < range) {
+        uint32_t thr = (uint32_t)(0u - range) % range;
+        while (low < thr) {
+            prod = (uint64_t)mt_next(r) * (uint64_t)range;
+            low = (uint32_t)prod;
+        }
+    }
+    return a + (int)(prod >> 32);
+}
+
+// ---------------------------------------------------------------------------
+// operator codes: orb | spin << 4 | (ca-1) << 5 ; flavour f = orb + nb*spin
+// ---------------------------------------------------------------------------
+DEV int op_orb(int c) { return c & 15; }
+DEV int op_spin(int c) { return (c >> 4) & 1; }
+DEV int op_ca(int c) { return ((c >> 5) & 1) + 1; }
+DEV int mk_code(int orb, int spin, int ca) { return orb | (spin << 4) | ((ca - 1) << 5); }
+
+// ---------------------------------------------------------------------------
+// chain context: LDS views of the header, model tables and work buffers
+// ---------------------------------------------------------------------------
+struct Ctx {
+    const DevModel &M;
+    const ChainLayout &L;
+    char *base;          // chain block in global memory
+    ChainHdr &h;         // LDS copy
+    Rng rng;
+    // model tables (LDS)
+    const int *sz, *off, *cn, *oo;
+    const double *ev, *opsp;   // opsp: operator blocks (LDS or global)
+    // chain arrays (global)
+    double *tau;
+    int *code;
+    double *fc, *fa, *minv, *scratch, *kst, *bnd, *bndlog, *part;
+    int *fcs, *fas;      // physical row (creator) / column (annihilator) slot of each rank
+    int *bndsst;
+    // work buffers
+    double *wtau;        // operator list (LDS), lcap entries
+    int *wcode;
+    double *K0, *K1;     // trace work matrices (LDS for small blocks)
+    double *dvec, *Q, *R, *colb, *rowb, *pl;
+    int nm;              // padded block size used for matrix strides
+    DEV Ctx(const DevModel &m, const ChainLayout &l, char *b, ChainHdr &hd) : M(m), L(l), base(b), h(hd) {}
+    DEV double *mi(int f) { return minv + (long)f * L.kcap * L.kcap; }
+    DEV int fl(int c) const { return op_orb(c) + M.nb * op_spin(c); }
+    DEV int conn(int s, int c, bool adjoint) const
+    {
+        int ca = op_ca(c);
+        if (adjoint) ca = 3 - ca;
+        return cn[((s * M.nb + op_orb(c)) * 2 + op_spin(c)) * 2 + (ca - 1)];
+    }
+    DEV const double *opblock(int s, int c, bool adjoint) const
+    {
+        int ca = op_ca(c);
+        if (adjoint) ca = 3 - ca;
+        return opsp + oo[((s * M.nb + op_orb(c)) * 2 + op_spin(c)) * 2 + (ca - 1)];
+    }
+};
+
+// get_HybrF (Trace.F90:1014-1033)
+DEV double hybr(const DevModel &M, double tau, int f)
+{
+    double sig = 1.0;
+    if (tau < 0.0) {
+        tau = tau + M.beta;
+        sig = -1.0;
+    }
+    double n = tau / M.beta * (double)(M.nftau - 1);
+    int i = (int)n;
+    const double *F = M.ftau + (long)f * M.nftau;
+    return sig * (F[i] + (n - (double)i) * (F[i + 1] - F[i]));
+}
+
+// ---------------------------------------------------------------------------
+// window (Trace.F90:1608-1751)
+// ---------------------------------------------------------------------------
+DEV void set_window(Ctx &c)
+{
+    if (c.h.nwin > 1) {
+        c.h.tauwin_min = ((double)c.h.wpos / (double)(c.h.nwin + 1)) * c.M.beta;
+        c.h.tauwin_max = ((double)(c.h.wpos + 2) / (double)(c.h.nwin + 1)) * c.M.beta;
+    }
+    c.h.wvalid = 0;
+}
+
+DEV void shift_window(Ctx &c)
+{
+    if (c.h.nwin > 1) {
+        c.h.wpos++;
+        if (c.h.wpos == c.h.nwin) c.h.wpos = 0;
+        set_window(c);
+    } else {
+        c.h.wpos = 0;
+    }
+}
+
+DEV double bound_tau(int m, int nwin, double beta)
+{
+    if (nwin <= 1) return m == 0 ? 0.0 : beta;
+    return ((double)m / (double)(nwin + 1)) * beta;
+}
+
+DEV int lower_bound_tau(const double *tau, int n, double x)
+{
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (tau[mid] < x) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+DEV int upper_bound_tau(const double *tau, int n, double x)
+{
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (tau[mid] <= x) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// indices [wia, wib) of the operators inside [tauwin_min, tauwin_max]
+DEV void ensure_window_range(Ctx &c)
+{
+    if (c.h.wvalid) return;
+    c.h.wia = lower_bound_tau(c.tau, c.h.nop, c.h.tauwin_min);
+    c.h.wib = upper_bound_tau(c.tau, c.h.nop, c.h.tauwin_max);
+    c.h.wvalid = 1;
+}
+
+// ---------------------------------------------------------------------------
+// local trace
+// ---------------------------------------------------------------------------
+// algorithmic work of one operator application on `no` outer states: a
+// (nt x ns) block times an (ns x no) state matrix plus the diagonal propagator;
+// bytes = operator block + states in and out.  Accumulated in registers during
+// a walk and added to the chain header once.
+struct Work {
+    double flops = 0.0, bytes = 0.0;
+    DEV void op(int nt, int ns, int no)
+    {
+        flops += (double)no * (2.0 * nt * ns + ns);
+        bytes += 8.0 * nt * ns + 8.0 * no * (ns + nt);
+    }
+    DEV void commit(Ctx &c) const
+    {
+        c.h.alg_flops += flops;
+        c.h.alg_bytes += bytes;
+    }
+};
+
+// exp(-(E_k - E0) dt) of superstate s into LDS dvec; returns the log factor
+// -(E0 - Egs) dt.  Ends with a barrier.  (Blocks above 16 states.)
+DEV double prop_factors(Ctx &c, int s, double dt)
+{
+    const int ns = c.sz[s];
+    const double *E = c.ev + c.off[s];
+    const double e0 = E[0];
+    for (int k = lane_id(); k < ns; k += WAVE) c.dvec[k] = (ns > 1) ? exp(-(E[k] - e0) * dt) : 1.0;
+    wsync();
+    return -(e0 - c.M.egs) * dt;
+}
+
+// Lane k's propagator factor exp(-(E_k - E0) dt) of superstate s (1 beyond the
+// block); the common log factor -(E0 - Egs) dt is added to lg.  No LDS traffic:
+// the factors are broadcast by readlane inside apply_small.
+DEV double prop_lane(const Ctx &c, int s, double dt, double &lg)
+{
+    const int ns = c.sz[s];
+    const double *E = c.ev + c.off[s];
+    const double e0 = E[0];
+    lg += -(e0 - c.M.egs) * dt;
+    const int k = lane_id();
+    const double ek = k < ns ? E[k] : e0;
+    return ns > 1 ? exp(-(ek - e0) * dt) : 1.0;
+}
+
+// Thresholds outside which a state matrix is renormalised during a walk; in
+// between, the entries are left unscaled (rescaling by powers of two is exact,
+// so skipping it changes no rounding).
+constexpr double STATE_TINY = 0x1p-64, STATE_HUGE = 0x1p64;
+
+// Output element u of this lane in a (rows x no) row-major state: row rr[u],
+// column ii[u]; computed once per walk.
+template <int NM>
+struct LaneSlots {
+    static constexpr int PER = (NM * NM + WAVE - 1) / WAVE;
+    int rr[PER], ii[PER];
+    DEV explicit LaneSlots(int no)
+    {
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int idx = lane_id() + u * WAVE;
+            rr[u] = idx / no;
+            ii[u] = idx - rr[u] * no;
+        }
+    }
+};
+
+// One operator on blocks of at most 16 states: Kout (nt x no) = P * diag(d) *
+// Kin (ns x no), d broadcast from the lanes of prop_lane.  One barrier.
+template <int NM>
+DEV void apply_small(const double *P, const double *Kin, double *Kout, int nt, int ns, int no, double d,
+                     const LaneSlots<NM> &ls, double &lg)
+{
+    constexpr int PER = LaneSlots<NM>::PER;
+    double dk[NM];
+#pragma unroll
+    for (int k = 0; k < NM; ++k) dk[k] = readlane_d(d, k);
+    double v[PER];
+    double m = 0.0;
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        double acc = 0.0;
+        const int r = ls.rr[u], i = ls.ii[u];
+        if (r < nt) {
+#pragma unroll
+            for (int k = 0; k < NM; ++k)
+                if (k < ns) acc = fma(P[r + k * nt] * dk[k], Kin[k * no + i], acc);
+        }
+        v[u] = acc;
+        m = fmax(m, fabs(acc));
+    }
+    if (__ballot(m >= STATE_TINY) == 0ull || __ballot(m > STATE_HUGE) != 0ull) {
+        const double mm = wave_max(m);
+        if (mm > 0.0) {
+            const int e = ilogb(mm);
+            lg += (double)e * 0.69314718055994530942;
+#pragma unroll
+            for (int u = 0; u < PER; ++u) v[u] = ldexp(v[u], -e);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < PER; ++u)
+        if (ls.rr[u] < nt) Kout[lane_id() + u * WAVE] = v[u];
+    wsync();
+}
+
+// Final propagation of a walk (rows scaled by lane r's factor d) and
+// normalisation to a maximum modulus of one.
+template <int NM>
+DEV void finish_small(double *K, int ns, int no, double d, const LaneSlots<NM> &ls, double &lg)
+{
+    constexpr int PER = LaneSlots<NM>::PER;
+    double v[PER];
+    double m = 0.0;
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int r = ls.rr[u];
+        const double dr = __shfl(d, r < WAVE ? r : 0, WAVE);
+        v[u] = r < ns ? K[lane_id() + u * WAVE] * dr : 0.0;
+        m = fmax(m, fabs(v[u]));
+    }
+    m = wave_max(m);
+    if (m > 0.0) {
+        const double inv = 1.0 / m;
+        lg += log(m);
+#pragma unroll
+        for (int u = 0; u < PER; ++u) v[u] *= inv;
+    }
+#pragma unroll
+    for (int u = 0; u < PER; ++u)
+        if (ls.rr[u] < ns) K[lane_id() + u * WAVE] = v[u];
+    wsync();
+}
+
+// One operator, blocks above 16 states: LDS propagator vector, rescaled to a
+// maximum modulus of one.
+template <int NM>
+DEV void apply_op(Ctx &c, const double *P, const double *Kin, double *Kout, int nt, int ns, int no,
+                  double &lg)
+{
+    const int nout = nt * no;
+    double m = 0.0;
+    for (int idx = lane_id(); idx < nout; idx += WAVE) {
+        int r = idx / no, i = idx - r * no;
+        double acc = 0.0;
+        for (int k = 0; k < ns; ++k) acc = fma(P[r + (long)k * nt] * c.dvec[k], Kin[k * no + i], acc);
+        Kout[idx] = acc;
+        m = fmax(m, fabs(acc));
+    }
+    m = wave_max(m);
+    wsync();
+    if (m > 0.0) {
+        double inv = 1.0 / m;
+        lg += log(m);
+        for (int idx = lane_id(); idx < nout; idx += WAVE) Kout[idx] *= inv;
+    }
+    wsync();
+}
+
+// scale rows by dvec and renormalise (end-of-sweep propagation)
+DEV void scale_rows(Ctx &c, double *K, int ns, int no, double &lg)
+{
+    double m = 0.0;
+    for (int idx = lane_id(); idx < ns * no; idx += WAVE) {
+        double x = K[idx] * c.dvec[idx / no];
+        K[idx] = x;
+        m = fmax(m, fabs(x));
+    }
+    m = wave_max(m);
+    wsync();
+    if (m > 0.0) {
+        double inv = 1.0 / m;
+        lg += log(m);
+        for (int idx = lane_id(); idx < ns * no; idx += WAVE) K[idx] *= inv;
+    }
+    wsync();
+}
+
+// Ket sweep over a list (ascending tau) from t0 to t1.  Returns false if an
+// operator has no target block (quantum-number violation).
+template <int NM>
+DEV bool ket_walk(Ctx &c, double *&K, double *&Kw, int &s, double &lg, int no, double t0,
+                  const double *otau, const int *ocode, int n, double t1)
+{
+    double t = t0;
+    Work w;
+    if constexpr (NM <= 16) {
+        const LaneSlots<NM> ls(no);
+        for (int j = 0; j < n; ++j) {
+            const double tj = otau[j];
+            const int cj = ocode[j];
+            const int ts = c.conn(s, cj, false);
+            if (ts < 0) return w.commit(c), false;
+            const double d = prop_lane(c, s, tj - t, lg);
+            const int ns = c.sz[s], nt = c.sz[ts];
+            w.op(nt, ns, no);
+            apply_small<NM>(c.opblock(s, cj, false), K, Kw, nt, ns, no, d, ls, lg);
+            double *x = K;
+            K = Kw;
+            Kw = x;
+            s = ts;
+            t = tj;
+        }
+        const double d = prop_lane(c, s, t1 - t, lg);
+        finish_small<NM>(K, c.sz[s], no, d, ls, lg);
+    } else {
+        for (int j = 0; j < n; ++j) {
+            const double tj = otau[j];
+            const int cj = ocode[j];
+            const int ts = c.conn(s, cj, false);
+            if (ts < 0) return w.commit(c), false;
+            lg += prop_factors(c, s, tj - t);
+            const int ns = c.sz[s], nt = c.sz[ts];
+            w.op(nt, ns, no);
+            apply_op<NM>(c, c.opblock(s, cj, false), K, Kw, nt, ns, no, lg);
+            double *x = K;
+            K = Kw;
+            Kw = x;
+            s = ts;
+            t = tj;
+        }
+        lg += prop_factors(c, s, t1 - t);
+        scale_rows(c, K, c.sz[s], no, lg);
+    }
+    w.commit(c);
+    return true;
+}
+
+// Bra sweep downward from t1 over a list (taken in descending order) to t0.
+template <int NM>
+DEV void bra_walk(Ctx &c, double *&B, double *&Bw, int &s, double &lg, int no, double t1,
+                  const double *otau, const int *ocode, int n, double t0)
+{
+    double t = t1;
+    Work w;
+    if constexpr (NM <= 16) {
+        const LaneSlots<NM> ls(no);
+        for (int j = n - 1; j >= 0; --j) {
+            const double tj = otau[j];
+            const int cj = ocode[j];
+            const int ts = c.conn(s, cj, true);
+            const double d = prop_lane(c, s, t - tj, lg);
+            const int ns = c.sz[s], nt = c.sz[ts];
+            w.op(nt, ns, no);
+            apply_small<NM>(c.opblock(s, cj, true), B, Bw, nt, ns, no, d, ls, lg);
+            double *x = B;
+            B = Bw;
+            Bw = x;
+            s = ts;
+            t = tj;
+        }
+        const double d = prop_lane(c, s, t - t0, lg);
+        finish_small<NM>(B, c.sz[s], no, d, ls, lg);
+    } else {
+        for (int j = n - 1; j >= 0; --j) {
+            const double tj = otau[j];
+            const int cj = ocode[j];
+            const int ts = c.conn(s, cj, true);
+            lg += prop_factors(c, s, t - tj);
+            const int ns = c.sz[s], nt = c.sz[ts];
+            w.op(nt, ns, no);
+            apply_op<NM>(c, c.opblock(s, cj, true), B, Bw, nt, ns, no, lg);
+            double *x = B;
+            B = Bw;
+            Bw = x;
+            s = ts;
+            t = tj;
+        }
+        lg += prop_factors(c, s, t - t0);
+        scale_rows(c, B, c.sz[s], no, lg);
+    }
+    w.commit(c);
+}
+
+// stage global operators [j0, j1) into the LDS list (chunks of lcap)
+DEV int stage_chunk(Ctx &c, int j0, int j1)
+{
+    const int n = min(j1 - j0, c.L.lcap_lds);
+    for (int j = lane_id(); j < n; j += WAVE) {
+        c.wtau[j] = c.tau[j0 + j];
+        c.wcode[j] = c.code[j0 + j];
+    }
+    wsync();
+    return n;
+}
+
+// ket over global operators [j0, j1) in LDS-staged chunks
+template <int NM>
+DEV bool ket_walk_global(Ctx &c, double *&K, double *&Kw, int &s, double &lg, int no, double t0, int j0,
+                         int j1, double t1)
+{
+    double t = t0;
+    for (int b = j0; b < j1;) {
+        int n = stage_chunk(c, b, j1);
+        double tend = (b + n < j1) ? c.wtau[n - 1] : t1;
+        // walk the chunk; the final propagation of a non-last chunk is zero
+        if (!ket_walk<NM>(c, K, Kw, s, lg, no, t, c.wtau, c.wcode, n, tend)) return false;
+        t = tend;
+        b += n;
+        wsync();
+    }
+    if (j0 >= j1) ket_walk<NM>(c, K, Kw, s, lg, no, t0, c.wtau, c.wcode, 0, t1);
+    return true;
+}
+
+template <int NM>
+DEV void bra_walk_global(Ctx &c, double *&B, double *&Bw, int &s, double &lg, int no, double t1, int j0,
+                         int j1, double t0)
+{
+    double t = t1;
+    for (int e = j1; e > j0;) {
+        int b = max(j0, e - c.L.lcap_lds);
+        int n = stage_chunk(c, b, e);
+        double tend = (b > j0) ? c.wtau[0] : t0;
+        bra_walk<NM>(c, B, Bw, s, lg, no, t, c.wtau, c.wcode, n, tend);
+        t = tend;
+        e = b;
+        wsync();
+    }
+    if (j0 >= j1) bra_walk<NM>(c, B, Bw, s, lg, no, t1, c.wtau, c.wcode, 0, t0);
+}
+
+DEV void set_identity(double *K, int no)
+{
+    for (int idx = lane_id(); idx < no * no; idx += WAVE) K[idx] = (idx / no == idx % no) ? 1.0 : 0.0;
+    wsync();
+}
+
+DEV void copy_mat(double *dst, const double *src, int n)
+{
+    for (int i = lane_id(); i < n; i += WAVE) dst[i] = src[i];
+    wsync();
+}
+
+// signed log-sum (addLogTr, Trace.F90:286-298)
+DEV void log_add(double &al, double &as, double bl, double bs)
+{
+    if (al >= bl) {
+        al = al + log(1.0 + as * bs * exp(bl - al));
+    } else {
+        double nl = bl + log(1.0 + as * bs * exp(al - bl));
+        al = nl;
+        as = bs;
+    }
+}
+
+// Contract ket K and bra B (same superstate s, both at the same tau) column by
+// column; part traces to c.pl, total in (tl, ts).
+DEV void contract(Ctx &c, const double *K, const double *B, int s, int no, double lg, double &tl, double &ts)
+{
+    const int ns = c.sz[s];
+    for (int i = lane_id(); i < no; i += WAVE) {
+        double acc = 0.0;
+        for (int r = 0; r < ns; ++r) acc = fma(K[r * no + i], B[r * no + i], acc);
+        c.pl[2 * i] = lg + log(fabs(acc));
+        c.pl[2 * i + 1] = copysign(1.0, acc);
+    }
+    wsync();
+    tl = -DBL_MAX;
+    ts = 0.0;
+    for (int i = 0; i < no; ++i) log_add(tl, ts, c.pl[2 * i], c.pl[2 * i + 1]);
+}
+
+// Full trace of the current configuration (global moves, tau shifts,
+// set_config, initial state): identity ket on the outer superstate at tau = 0
+// through every operator, contracted with the identity bra at beta.
+template <int NM>
+DEV void full_trace(Ctx &c, int outer, double &tl, double &ts)
+{
+    const int no = c.sz[outer];
+    double *K = c.K0, *Kw = c.K1;
+    set_identity(K, no);
+    int s = outer;
+    double lg = 0.0;
+    ket_walk_global<NM>(c, K, Kw, s, lg, no, 0.0, 0, c.h.nop, c.M.beta);
+    set_identity(Kw, no);
+    contract(c, K, Kw, s, no, lg, tl, ts);
+}
+
+// ---------------------------------------------------------------------------
+// boundary cache: ket at the window's lower edge, bras at every upper edge
+// ---------------------------------------------------------------------------
+template <int NM>
+DEV void rebuild_cache(Ctx &c)
+{
+    const int outer = c.h.outer, no = c.sz[outer];
+    const int p = c.h.nwin > 1 ? c.h.wpos : 0;
+    const int nw = c.h.nwin;
+    const int top = (nw > 1) ? nw + 1 : 2;
+    const int stride = c.nm * c.nm;
+    double *B = c.K0, *Bw = c.K1;
+    set_identity(B, no);
+    int s = outer;
+    double lg = 0.0;
+    double tb = c.M.beta;
+    int hiidx = c.h.nop;
+    copy_mat(c.bnd + (long)top * stride, B, no * no);
+    if (lane_id() == 0) {
+        c.bndlog[top] = lg;
+        c.bndsst[top] = s;
+    }
+    for (int m = top - 1; m >= p + 2; --m) {
+        const double tm = bound_tau(m, nw, c.M.beta);
+        const int loidx = upper_bound_tau(c.tau, c.h.nop, tm);
+        bra_walk_global<NM>(c, B, Bw, s, lg, no, tb, loidx, hiidx, tm);
+        copy_mat(c.bnd + (long)m * stride, B, c.sz[s] * no);
+        if (lane_id() == 0) {
+            c.bndlog[m] = lg;
+            c.bndsst[m] = s;
+        }
+        tb = tm;
+        hiidx = loidx;
+    }
+    double *K = c.K0, *Kw = c.K1;
+    set_identity(K, no);
+    s = outer;
+    lg = 0.0;
+    const double tp = bound_tau(p, nw, c.M.beta);
+    const int n0 = lower_bound_tau(c.tau, c.h.nop, tp);
+    ket_walk_global<NM>(c, K, Kw, s, lg, no, 0.0, 0, n0, tp);
+    copy_mat(c.kst, K, c.sz[s] * no);
+    c.h.kst_log = lg;
+    c.h.kst_sst = s;
+    c.h.cache_pos = p;
+    wsync();
+}
+
+// Bring the caches to the current window position: advance the ket across
+// passed half windows, or rebuild after a wrap / invalidation.
+template <int NM>
+DEV void ensure_cache(Ctx &c)
+{
+    const int p = c.h.nwin > 1 ? c.h.wpos : 0;
+    if (c.h.cache_pos == p) return;
+    if (c.h.cache_pos < 0 || c.h.cache_pos > p) {
+        rebuild_cache<NM>(c);
+        return;
+    }
+    const int no = c.sz[c.h.outer];
+    int s = c.h.kst_sst;
+    double lg = c.h.kst_log;
+    double *K = c.K0, *Kw = c.K1;
+    copy_mat(K, c.kst, c.sz[s] * no);
+    const double ta = bound_tau(c.h.cache_pos, c.h.nwin, c.M.beta);
+    const double tb = bound_tau(p, c.h.nwin, c.M.beta);
+    const int ia = lower_bound_tau(c.tau, c.h.nop, ta);
+    const int ib = lower_bound_tau(c.tau, c.h.nop, tb);
+    ket_walk_global<NM>(c, K, Kw, s, lg, no, ta, ia, ib, tb);
+    copy_mat(c.kst, K, c.sz[s] * no);
+    c.h.kst_log = lg;
+    c.h.kst_sst = s;
+    c.h.cache_pos = p;
+    wsync();
+}
+
+// Trace of the LDS trial list between the cached ket and bra; returns false
+// on a quantum-number violation (check_sst_sequence, Trace.F90:1243-1302, on a
+// superset of the changed span).
+template <int NM>
+DEV bool window_trace(Ctx &c, int nl, double &tl, double &ts)
+{
+    const int no = c.sz[c.h.outer];
+    const int p = c.h.nwin > 1 ? c.h.wpos : 0;
+    const int top = p + 2;
+    const double ta = bound_tau(p, c.h.nwin, c.M.beta), tb = bound_tau(top, c.h.nwin, c.M.beta);
+    double *K = c.K0, *Kw = c.K1;
+    int s = c.h.kst_sst;
+    copy_mat(K, c.kst, c.sz[s] * no);
+    double lg = c.h.kst_log;
+    if (!ket_walk<NM>(c, K, Kw, s, lg, no, ta, c.wtau, c.wcode, nl, tb)) return false;
+    if (s != c.bndsst[top]) return false;
+    contract(c, K, c.bnd + (long)top * c.nm * c.nm, s, no, lg + c.bndlog[top], tl, ts);
+    return true;
+}
+
+// load the window's operators [wia, wib) into the LDS list
+DEV void load_window(Ctx &c)
+{
+    const int ia = c.h.wia, nw = c.h.wib - c.h.wia;
+    for (int j = lane_id(); j < nw; j += WAVE) {
+        c.wtau[j] = c.tau[ia + j];
+        c.wcode[j] = c.code[ia + j];
+    }
+    wsync();
+}
+
+// ---------------------------------------------------------------------------
+// removable pairs (ensure_valid_NPairs_diag, Trace.F90:1769-1810) in a list
+// ---------------------------------------------------------------------------
+DEV int pairs_from(const Ctx &c, const double *tau, const int *code, int n, int j)
+{
+    const double te = tau[j];
+    if (te < c.h.tauwin_min || te > c.h.tauwin_max) return 0;
+    const int ce = code[j];
+    const int fe = c.fl(ce), ae = op_ca(ce);
+    int cnt = 0;
+    for (int k = j + 1; k < n; ++k) {
+        const double tk = tau[k];
+        if (tk > c.h.tauwin_max || tk > te + c.h.taudiff_max) break;
+        const int ck = code[k];
+        if (c.fl(ck) == fe && op_ca(ck) != ae) ++cnt;
+    }
+    return cnt;
+}
+
+DEV int npairs_list(const Ctx &c, const double *tau, const int *code, int n)
+{
+    int cnt = 0;
+    for (int j = lane_id(); j < n; j += WAVE) cnt += pairs_from(c, tau, code, n, j);
+    return wave_isum(cnt);
+}
+
+// ---------------------------------------------------------------------------
+// configuration sign: get_Sign with get_BlockSign and get_permaltca
+// (Trace.F90:3479-3615).  Parities of
+//   BlockSign: sum over operators j of the counts of later operators of the
+//     flavours (b', s') with b' < o_j, plus (o_j, s'=0) when s_j = 1;
+//   permaltca: sum over creators of the count of later same-flavour operators,
+//     plus 1 for an odd number of pairs;
+// computed per flavour g with ballots over chunks of 64 operators.
+// ---------------------------------------------------------------------------
+DEV double config_sign(Ctx &c)
+{
+    const int n = c.h.nop, nb = c.M.nb, nf = c.M.nf;
+    // stage the operator codes in LDS (the list buffer is free here)
+    const int *codes = c.code;
+    if (n <= 2 * c.L.lcap_lds) {
+        int *st = reinterpret_cast<int *>(c.wtau);
+        for (int j = lane_id(); j < n; j += WAVE) st[j] = c.code[j];
+        wsync();
+        codes = st;
+    }
+    int parity = 0, alt = 0;
+    for (int g = 0; g < nf; ++g) {
+        const int bg = g % nb, sg = g / nb;
+        const int mg = 2 * c.h.kf[g];
+        int before = 0;   // flavour-g operators before the current chunk
+        for (int b0 = 0; b0 < n; b0 += WAVE) {
+            const int j = b0 + lane_id();
+            int cj = 0;
+            bool isg = false;
+            if (j < n) {
+                cj = codes[j];
+                isg = c.fl(cj) == g;
+            }
+            const unsigned long long mask = __ballot(isg);
+            if (j < n) {
+                const int upto = before + popc64(mask & lanemask_le());   // flavour g up to j
+                const int later = mg - upto;
+                const int oj = op_orb(cj), sj = op_spin(cj);
+                const bool in_set = (bg < oj) || (sj == 1 && bg == oj && sg == 0);
+                if (in_set) parity ^= later & 1;
+                if (isg && op_ca(cj) == 1) alt ^= later & 1;
+            }
+            before += popc64(mask);
+        }
+    }
+    wsync();
+    parity = wave_isum(parity) & 1;
+    alt = wave_isum(alt) & 1;
+    if ((n / 2) % 2 == 1) alt ^= 1;
+    const double block = parity ? -1.0 : 1.0, altsign = alt ? -1.0 : 1.0;
+    return c.h.trace_sign * c.h.det_sign * block * altsign;
+}
+
+// ---------------------------------------------------------------------------
+// hybridisation matrix algebra (MatrixUpdate.F90)
+// ---------------------------------------------------------------------------
+// S - R^T M Q for an order-k block (get_DetRatAdd)
+DEV double detrat_add(Ctx &c, const double *Mf, int k, double S)
+{
+    const int ld = c.L.kcap;
+    double part = 0.0;
+    for (int i = lane_id(); i < k; i += WAVE) {
+        double v = 0.0;
+#pragma unroll 8
+        for (int j = 0; j < k; ++j) v = fma(Mf[i + (long)j * ld], c.Q[j], v);
+        part = fma(c.R[i], v, part);
+    }
+    return S - wave_sum(part);
+}
+
+// Bordering update to order k+1, get_MatAdd (MatrixUpdate.F90:410-442), on
+// physical storage: the new creator takes physical row k, the new annihilator
+// physical column k, so no entry moves.  c.Q holds Q by physical column, c.R
+// holds R by physical row.
+//   col(i) = -(M Q)_i / dr,  row(j) = -(R M)_j / dr,
+//   M(i,j) += dr col(i) row(j),  M(i,k) = col(i),  M(k,j) = row(j),  M(k,k) = 1/dr
+DEV void mat_add_phys(Ctx &c, double *Mf, int k, double dr)
+{
+    const int ld = c.L.kcap;
+    const int l = lane_id();
+    double *col = c.colb, *row = c.rowb;
+    for (int i = l; i < k; i += WAVE) {
+        double v = 0.0;
+#pragma unroll 8
+        for (int j = 0; j < k; ++j) v = fma(Mf[i + (long)j * ld], c.Q[j], v);
+        col[i] = -v / dr;
+    }
+    for (int j = l; j < k; j += WAVE) {
+        double v = 0.0;
+        const double *mc = Mf + (long)j * ld;
+#pragma unroll 8
+        for (int i = 0; i < k; ++i) v = fma(c.R[i], mc[i], v);
+        row[j] = -v / dr;
+    }
+    wsync();
+    for (int idx = l; idx < k * k; idx += WAVE) {
+        const int i = idx % k, j = idx / k;
+        Mf[i + (long)j * ld] = fma(dr * col[i], row[j], Mf[i + (long)j * ld]);
+    }
+    for (int i = l; i < k; i += WAVE) {
+        Mf[i + (long)k * ld] = col[i];
+        Mf[k + (long)i * ld] = row[i];
+    }
+    if (l == 0) Mf[k + (long)k * ld] = 1.0 / dr;
+    wsync();
+}
+
+// Removal of the creator in physical row sc and the annihilator in physical
+// column sa, get_MatRem (MatrixUpdate.F90:455-482):
+//   M(i,j) -= M(i,sa) M(sc,j) / M(sc,sa);
+// then the last physical row / column move into the freed slots.
+DEV void mat_rem_phys(Ctx &c, double *Mf, int k, int sc, int sa)
+{
+    const int ld = c.L.kcap;
+    const int l = lane_id();
+    double *colp = c.colb, *rowp = c.rowb;
+    for (int i = l; i < k; i += WAVE) {
+        colp[i] = Mf[i + (long)sa * ld];
+        rowp[i] = Mf[sc + (long)i * ld];
+    }
+    wsync();
+    const double inv = 1.0 / rowp[sa];
+    for (int idx = l; idx < k * k; idx += WAVE) {
+        const int i = idx % k, j = idx / k;
+        Mf[i + (long)j * ld] = fma(-colp[i] * inv, rowp[j], Mf[i + (long)j * ld]);
+    }
+    wsync();
+    const int last = k - 1;
+    if (sc != last) {
+        for (int j = l; j < k; j += WAVE) Mf[sc + (long)j * ld] = Mf[last + (long)j * ld];
+        wsync();
+    }
+    if (sa != last) {
+        for (int i = l; i < k; i += WAVE) Mf[i + (long)sa * ld] = Mf[i + (long)last * ld];
+        wsync();
+    }
+}
+
+// LU with partial pivoting of A (k x k, ld) in place; log|det| and sign as
+// get_LogDetFull (MatrixUpdate.F90:302-339).  Returns false if singular.
+DEV bool lu_inplace(double *A, int k, int ld, double &dl, double &ds, int *piv)
+{
+    const int l = lane_id();
+    dl = 0.0;
+    ds = 1.0;
+    for (int col = 0; col < k; ++col) {
+        double best = -1.0;
+        int bi = col;
+        for (int r = col + l; r < k; r += WAVE) {
+            double v = fabs(A[r + (long)col * ld]);
+            if (v > best) best = v, bi = r;
+        }
+        // largest modulus, lowest row among ties
+        const double wb = wave_max(best);
+        bi = wave_imin(best == wb ? bi : 0x7fffffff);
+        best = wb;
+        if (best == 0.0) return false;
+        if (l == 0) piv[col] = bi;
+        if (bi != col) {
+            for (int j = l; j < k; j += WAVE) {
+                double t = A[col + (long)j * ld];
+                A[col + (long)j * ld] = A[bi + (long)j * ld];
+                A[bi + (long)j * ld] = t;
+            }
+        }
+        wsync();
+        double d = A[col + (long)col * ld];
+        ds *= (bi != col ? -1.0 : 1.0) * (d >= 0 ? 1.0 : -1.0);
+        dl += log(fabs(d));
+        for (int r = col + 1 + l; r < k; r += WAVE) A[r + (long)col * ld] /= d;
+        wsync();
+        for (int j = col + 1; j < k; ++j) {
+            double u = A[col + (long)j * ld];
+            for (int r = col + 1 + l; r < k; r += WAVE) A[r + (long)j * ld] -= A[r + (long)col * ld] * u;
+        }
+        wsync();
+    }
+    return true;
+}
+
+// Inverse from an in-place LU into out (k x k, ld)
+DEV void lu_inverse(const double *LU, int k, int ld, const int *piv, double *out)
+{
+    const int l = lane_id();
+    for (int col = l; col < k; col += WAVE) {
+        double *x = out + (long)col * ld;
+        for (int i = 0; i < k; ++i) x[i] = (i == col) ? 1.0 : 0.0;
+        for (int i = 0; i < k; ++i) {
+            int p = piv[i];
+            double t = x[i];
+            x[i] = x[p];
+            x[p] = t;
+        }
+        for (int i = 0; i < k; ++i) {
+            double xi = x[i];
+            for (int r = i + 1; r < k; ++r) x[r] -= LU[r + (long)i * ld] * xi;
+        }
+        for (int i = k - 1; i >= 0; --i) {
+            double xi = x[i] / LU[i + (long)i * ld];
+            x[i] = xi;
+            for (int r = 0; r < i; ++r) x[r] -= LU[r + (long)i * ld] * xi;
+        }
+    }
+    wsync();
+}
+
+// Delta_f(row = annihilator rank, col = creator rank) = F(tau_a - tau_c)
+// (get_InvFullHybr, Trace.F90:2279-2333) from the per-flavour lists
+DEV void build_delta(Ctx &c, int f, int k, double *D, const double *fcl, const double *fal)
+{
+    const int ld = c.L.kcap;
+    const double *ct = fcl + (long)f * c.L.kcap, *at = fal + (long)f * c.L.kcap;
+    for (int idx = lane_id(); idx < k * k; idx += WAVE) {
+        int r = idx % k, col = idx / k;
+        D[r + (long)col * ld] = hybr(c.M, at[r] - ct[col], f);
+    }
+    wsync();
+}
+
+// ---------------------------------------------------------------------------
+// array edits (lane-parallel shifts; chunks ordered so nothing is overwritten
+// before it is read)
+// ---------------------------------------------------------------------------
+// Move arr[from, to) by `by` places (either sign).  Rounds of 8 elements per
+// lane: all loads of a round complete before its stores, and rounds run from
+// the end that is written into first, so nothing is overwritten unread.
+template <typename T>
+DEV void block_move(T *arr, int from, int to, int by)
+{
+    constexpr int U = 8, R = U * WAVE;
+    const int l = lane_id();
+    const int cnt = to - from;
+    if (cnt <= 0 || by == 0) return;
+    const int nr = (cnt + R - 1) / R;
+    for (int q = 0; q < nr; ++q) {
+        const int r = by > 0 ? nr - 1 - q : q;
+        const int b = from + r * R;
+        T v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int i = b + u * WAVE + l;
+            if (i < to) v[u] = arr[i];
+        }
+        wsync();
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int i = b + u * WAVE + l;
+            if (i < to) arr[i + by] = v[u];
+        }
+        wsync();
+    }
+}
+
+// parallel count of entries < x in a sorted list of n
+DEV int count_below_par(const double *arr, int n, double x)
+{
+    int cnt = 0;
+    for (int j = lane_id(); j < n; j += WAVE) cnt += arr[j] < x;
+    return wave_isum(cnt);
+}
+
+// ---------------------------------------------------------------------------
+// moves
+// ---------------------------------------------------------------------------
+enum { OUT_REJ = 0, OUT_ACC = 1, OUT_QN = 2 };
+enum { ERR_CAP = 3, ERR_SING = 4, ERR_STATE_DEV = 6 };
+
+DEV void store_part(Ctx &c, int no)
+{
+    for (int i = lane_id(); i < 2 * no; i += WAVE) c.part[i] = c.pl[i];
+    wsync();
+}
+
+// write the LDS trial list into the global operator list at wia (its length
+// nl replaces the window's nw operators: the tail [wib, nop) moves first)
+DEV void commit_window(Ctx &c, int nl)
+{
+    const int nw = c.h.wib - c.h.wia;
+    block_move(c.tau, c.h.wib, c.h.nop, nl - nw);
+    block_move(c.code, c.h.wib, c.h.nop, nl - nw);
+    for (int j = lane_id(); j < nl; j += WAVE) {
+        c.tau[c.h.wia + j] = c.wtau[j];
+        c.code[c.h.wia + j] = c.wcode[j];
+    }
+    wsync();
+    c.h.nop += nl - nw;
+    c.h.wib = c.h.wia + nl;
+}
+
+// StepAdd (CTQMC.F90:3353-3612), change_outer = false
+template <int NM>
+DEV int step_add(Ctx &c)
+{
+    const DevModel &M = c.M;
+    c.h.try_add++;
+    const double width = c.h.tauwin_max - c.h.tauwin_min;
+    const double tref = c.h.tauwin_min + width * grnd(c.rng);
+    double tdf = width;
+    const int orb = randint(c.rng, 1, M.nb) - 1;
+    const int spin = randint(c.rng, 1, 2) - 1;
+    const double tmin = fmax(tref - c.h.taudiff_max, c.h.tauwin_min);
+    const double w2 = fmin(tref + c.h.taudiff_max, c.h.tauwin_max) - tmin;
+    const double t2 = tmin + w2 * grnd(c.rng);
+    tdf *= w2;
+    const int f = orb + M.nb * spin;
+    if (t2 == tref) return c.h.qn_rej++, OUT_QN;
+    const int k = c.h.kf[f];
+    ensure_window_range(c);
+    const int nw = c.h.wib - c.h.wia;
+    if (k + 1 >= c.L.kcap || c.h.nop + 2 > c.L.cap || nw + 2 > c.L.lcap_lds) {
+        c.h.err = ERR_CAP;
+        return OUT_REJ;
+    }
+    ensure_cache<NM>(c);
+    load_window(c);
+    // equal times (process_OperAdd, Trace.F90:4287-4380) and insertion points
+    const double ta = tref, tc = t2;
+    const double t1 = fmin(ta, tc), t2o = fmax(ta, tc);
+    const int code_a = mk_code(orb, spin, 2), code_c = mk_code(orb, spin, 1);
+    const int code1 = ta < tc ? code_a : code_c, code2 = ta < tc ? code_c : code_a;
+    int eq = 0, q1 = 0, q2 = 0;
+    for (int j = lane_id(); j < nw; j += WAVE) {
+        const double x = c.wtau[j];
+        eq |= (x == t1) | (x == t2o);
+        q1 += x < t1;
+        q2 += x < t2o;
+    }
+    if (wave_isum(eq)) return c.h.qn_rej++, OUT_QN;
+    q1 = wave_isum(q1);
+    q2 = wave_isum(q2);
+    // trial list: [q2, nw) up by 2, [q1, q2) up by 1, new operators in the gaps
+    block_move(c.wtau, q2, nw, 2);
+    block_move(c.wcode, q2, nw, 2);
+    block_move(c.wtau, q1, q2, 1);
+    block_move(c.wcode, q1, q2, 1);
+    if (lane_id() == 0) {
+        c.wtau[q1] = t1;
+        c.wcode[q1] = code1;
+        c.wtau[q2 + 1] = t2o;
+        c.wcode[q2 + 1] = code2;
+    }
+    wsync();
+    const int nl = nw + 2;
+    double tl, ts;
+    if (!window_trace<NM>(c, nl, tl, ts)) return c.h.qn_rej++, OUT_QN;
+    // det ratio, get_LogDetRatPairAdd (Trace.F90:2775-2832): Q (annihilators) and
+    // R (creators) placed by physical slot; FPos = 1 + earlier operators of the type
+    const long fo = (long)f * c.L.kcap;
+    const double *ctl = c.fc + fo, *atl = c.fa + fo;
+    const int *csl = c.fcs + fo, *asl = c.fas + fo;
+    int fpc = 0, fpa = 0;
+    for (int j = lane_id(); j < k; j += WAVE) {
+        const double xa = atl[j], xc = ctl[j];
+        c.Q[asl[j]] = hybr(M, xa - tc, f);
+        c.R[csl[j]] = hybr(M, ta - xc, f);
+        fpa += xa < ta;
+        fpc += xc < tc;
+    }
+    const int fp_c = wave_isum(fpc) + 1, fp_a = wave_isum(fpa) + 1;
+    wsync();
+    const double S = detrat_add(c, c.mi(f), k, hybr(M, ta - tc, f));
+    c.h.alg_flops += 2.0 * k * k;
+    c.h.alg_bytes += 8.0 * k * k;
+    const double sig = (abs(fp_c - fp_a) % 2) ? -1.0 : 1.0;
+    const double dr_log = log(fabs(S)), dr_sign = copysign(1.0, S) * sig;
+    const double rnd = grnd(c.rng);
+    // NPairs of the new configuration: pairs with both ends in the window all
+    // lie in the trial list
+    const int np = npairs_list(c, c.wtau, c.wcode, nl);
+    const double wr = exp((tl - c.h.trace_log) + dr_log);
+    const double prob = fabs(((1.0 * tdf) * (double)(2 * M.nb)) / (double)np * wr);
+    if (!(rnd < prob)) return OUT_REJ;
+    // accept: M, operator list, flavour lists, weights, sign
+    mat_add_phys(c, c.mi(f), k, S);
+    c.h.alg_flops += 6.0 * k * k;
+    c.h.alg_bytes += 8.0 * k * k + 8.0 * (k + 1) * (k + 1);
+    commit_window(c, nl);
+    double *fcf = c.fc + fo, *faf = c.fa + fo;
+    int *fcsf = c.fcs + fo, *fasf = c.fas + fo;
+    block_move(fcf, fp_c - 1, k, 1);
+    block_move(fcsf, fp_c - 1, k, 1);
+    block_move(faf, fp_a - 1, k, 1);
+    block_move(fasf, fp_a - 1, k, 1);
+    if (lane_id() == 0) {
+        fcf[fp_c - 1] = tc;
+        fcsf[fp_c - 1] = k;
+        faf[fp_a - 1] = ta;
+        fasf[fp_a - 1] = k;
+    }
+    wsync();
+    c.h.kf[f] = k + 1;
+    c.h.det_log += dr_log;
+    c.h.det_sign *= dr_sign;
+    c.h.trace_log = tl;
+    c.h.trace_sign = ts;
+    store_part(c, c.sz[c.h.outer]);
+    c.h.cfgsign = -config_sign(c);
+    c.h.acc_add++;
+    return OUT_ACC;
+}
+
+// rank whose slot equals `slot` in a slot list of length k (-1 if none)
+DEV int find_slot(const int *sl, int k, int slot)
+{
+    int r = -1;
+    for (int j = lane_id(); j < k; j += WAVE)
+        if (sl[j] == slot) r = j;
+    return wave_imax(r);
+}
+
+// StepRem (CTQMC.F90:3892-4150) with gen_OperRemove (Trace.F90:4851-4977)
+template <int NM>
+DEV int step_rem(Ctx &c)
+{
+    const DevModel &M = c.M;
+    c.h.try_rem++;
+    ensure_window_range(c);
+    const int nw = c.h.wib - c.h.wia;
+    if (nw > c.L.lcap_lds) {
+        c.h.err = ERR_CAP;
+        return OUT_REJ;
+    }
+    ensure_cache<NM>(c);
+    load_window(c);
+    // pair counts per window operator (the window holds every removable pair)
+    int np = 0;
+    for (int j0 = 0; j0 < nw; j0 += WAVE) {
+        const int j = j0 + lane_id();
+        const int cnt = j < nw ? pairs_from(c, c.wtau, c.wcode, nw, j) : 0;
+        np += wave_isum(cnt);
+    }
+    if (np <= 0) return c.h.qn_rej++, OUT_QN;
+    int pick = randint(c.rng, 1, np);
+    int w1 = -1, w2 = -1;
+    {
+        int base = 0;
+        for (int j0 = 0; j0 < nw && w1 < 0; j0 += WAVE) {
+            const int j = j0 + lane_id();
+            const int cnt = j < nw ? pairs_from(c, c.wtau, c.wcode, nw, j) : 0;
+            int tot;
+            const int ex = wave_excl_scan(cnt, &tot);
+            int found = (pick > base + ex && pick <= base + ex + cnt) ? j : -1;
+            found = wave_imax(found);
+            if (found >= 0) {
+                int want = pick - base - __shfl(ex, found - j0, WAVE);
+                const int ce = c.wcode[found];
+                const int fe = c.fl(ce), ae = op_ca(ce);
+                int kk = found + 1;
+                for (; kk < nw; ++kk) {
+                    const int ck = c.wcode[kk];
+                    if (c.fl(ck) == fe && op_ca(ck) != ae)
+                        if (--want <= 0) break;
+                }
+                w1 = found;
+                w2 = kk;
+            }
+            base += tot;
+        }
+    }
+    if (w1 < 0 || w2 >= nw) {
+        c.h.err = ERR_STATE_DEV;
+        return OUT_REJ;
+    }
+    const int c1 = c.wcode[w1];
+    const double t1 = c.wtau[w1], t2 = c.wtau[w2];
+    const int f = c.fl(c1);
+    const int k = c.h.kf[f];
+    const double tc = op_ca(c1) == 1 ? t1 : t2, ta = op_ca(c1) == 1 ? t2 : t1;
+    const double hprf = (double)np /
+                        ((c.h.tauwin_max - c.h.tauwin_min) *
+                         (fmin(ta + c.h.taudiff_max, c.h.tauwin_max) - fmax(ta - c.h.taudiff_max, c.h.tauwin_min)));
+    // trial list without the pair
+    block_move(c.wtau, w1 + 1, w2, -1);
+    block_move(c.wcode, w1 + 1, w2, -1);
+    block_move(c.wtau, w2 + 1, nw, -2);
+    block_move(c.wcode, w2 + 1, nw, -2);
+    const int nl = nw - 2;
+    double tl, ts;
+    if (!window_trace<NM>(c, nl, tl, ts)) return c.h.qn_rej++, OUT_QN;
+    const long fo = (long)f * c.L.kcap;
+    const int fp_c = count_below_par(c.fc + fo, k, tc) + 1, fp_a = count_below_par(c.fa + fo, k, ta) + 1;
+    const int sc = c.fcs[fo + fp_c - 1], sa = c.fas[fo + fp_a - 1];
+    // get_LogDetRatPairRem (Trace.F90:2937-2963): M(FPos(1), FPos(2))
+    const int ld = c.L.kcap;
+    const double S = c.mi(f)[sc + (long)sa * ld];
+    const double sig = (abs(fp_c - fp_a) % 2) ? -1.0 : 1.0;
+    const double dr_log = log(fabs(S)), dr_sign = copysign(1.0, S) * sig;
+    const double wr = exp((tl - c.h.trace_log) + dr_log);
+    const double prob = fabs(((1.0 * hprf) / (double)(2 * M.nb)) * wr);
+    const double rnd = grnd(c.rng);
+    if (!(rnd < prob)) return OUT_REJ;
+    if (fabs(dr_sign * exp(dr_log)) < 1e-12) {
+        c.h.err = ERR_SING;
+        return OUT_REJ;
+    }
+    mat_rem_phys(c, c.mi(f), k, sc, sa);
+    c.h.alg_flops += 3.0 * k * k;
+    c.h.alg_bytes += 8.0 * k * k + 8.0 * (k - 1) * (k - 1);
+    // the operators that owned the last physical row / column take the freed slots
+    int *fcsf = c.fcs + fo, *fasf = c.fas + fo;
+    const int rl = find_slot(fcsf, k, k - 1), al = find_slot(fasf, k, k - 1);
+    if (lane_id() == 0) {
+        fcsf[rl] = sc;
+        fasf[al] = sa;
+    }
+    wsync();
+    commit_window(c, nl);
+    block_move(c.fc + fo, fp_c, k, -1);
+    block_move(fcsf, fp_c, k, -1);
+    block_move(c.fa + fo, fp_a, k, -1);
+    block_move(fasf, fp_a, k, -1);
+    c.h.kf[f] = k - 1;
+    c.h.det_log += dr_log;
+    c.h.det_sign *= dr_sign;
+    c.h.trace_log = tl;
+    c.h.trace_sign = ts;
+    store_part(c, c.sz[c.h.outer]);
+    c.h.cfgsign = -config_sign(c);
+    c.h.acc_rem++;
+    return OUT_ACC;
+}
+
+// relabel flavours / swap c and c+ (globalUpdate, Trace.F90:2126-2188)
+DEV void global_update(Ctx &c, const int *gu)
+{
+    const DevModel &M = c.M;
+    for (int j = lane_id(); j < c.h.nop; j += WAVE) {
+        const int cj = c.code[j];
+        const int idx = op_orb(cj) + M.nb * op_spin(cj) + 1;
+        const int g = gu[idx];
+        int ca = op_ca(cj);
+        if (gu[0] != 0) ca = 3 - ca;
+        c.code[j] = mk_code((g - 1) % M.nb, (g - 1) / M.nb, ca);
+    }
+    wsync();
+}
+
+// per-flavour tau lists and orders from the operator list: for every flavour,
+// ballot-compact the creators / annihilators of each 64-operator chunk.  With
+// slot lists given, slots are reset to the identity (logical = physical order).
+DEV void rebuild_flavour_lists(Ctx &c, double *dfc, double *dfa, int *dkf, int *dfcs, int *dfas)
+{
+    const DevModel &M = c.M;
+    const long kc = c.L.kcap;
+    for (int g = 0; g < M.nf; ++g) {
+        int nc = 0, na = 0;
+        for (int b0 = 0; b0 < c.h.nop; b0 += WAVE) {
+            const int j = b0 + lane_id();
+            int cj = 0;
+            double tj = 0.0;
+            if (j < c.h.nop) {
+                cj = c.code[j];
+                tj = c.tau[j];
+            }
+            const bool isc = j < c.h.nop && c.fl(cj) == g && op_ca(cj) == 1;
+            const bool isa = j < c.h.nop && c.fl(cj) == g && op_ca(cj) == 2;
+            const unsigned long long mc = __ballot(isc), ma = __ballot(isa);
+            const unsigned long long below = (lane_id() == 0) ? 0ull : (~0ull >> (64 - lane_id()));
+            if (isc) dfc[g * kc + nc + popc64(mc & below)] = tj;
+            if (isa) dfa[g * kc + na + popc64(ma & below)] = tj;
+            nc += popc64(mc);
+            na += popc64(ma);
+        }
+        if (dfcs)
+            for (int r = lane_id(); r < nc; r += WAVE) {
+                dfcs[g * kc + r] = r;
+                dfas[g * kc + r] = r;
+            }
+        if (lane_id() == 0) dkf[g] = nc;
+        wsync();
+    }
+    wsync();
+}
+
+DEV bool qn_cycle_global(Ctx &c, int s0)
+{
+    int s = s0;
+    for (int j = 0; j < c.h.nop && s >= 0; ++j) s = c.conn(s, c.code[j], false);
+    return s == s0;
+}
+
+// lane-parallel: which superstates give a valid cyclic sequence (lane = start
+// superstate, up to 64 per pass); bit s of the returned masks
+DEV int count_valid_outer(Ctx &c, unsigned long long *okmask)
+{
+    int total = 0;
+    for (int s0 = 0; s0 < c.M.nsst; s0 += WAVE) {
+        const int s = s0 + lane_id();
+        bool ok = false;
+        if (s < c.M.nsst) {
+            int cur = s;
+            for (int j = 0; j < c.h.nop && cur >= 0; ++j) cur = c.conn(cur, c.code[j], false);
+            ok = (cur == s);
+        }
+        const unsigned long long m = __ballot(ok);
+        okmask[s0 / WAVE] = m;
+        total += popc64(m);
+    }
+    return total;
+}
+
+// StepGlob (CTQMC.F90:4433-4554)
+template <int NM>
+DEV int step_glob(Ctx &c)
+{
+    const DevModel &M = c.M;
+    c.h.try_glob++;
+    if (c.h.nop == 0) return OUT_REJ;
+    const int nf = M.nf, nb = M.nb;
+    int gu[MAXF + 1], igu[MAXF + 1];
+    const double r = grnd(c.rng);
+    bool spinflip = false;
+    if (r < 0.3) {
+        spinflip = true;
+        gu[0] = 0;
+        for (int i = 1; i <= nb; ++i) gu[i] = i + nb;
+        for (int i = nb + 1; i <= 2 * nb; ++i) gu[i] = i - nb;
+    } else if (r < 0.6) {
+        gu[0] = 1;
+        for (int i = 1; i <= nf; ++i) gu[i] = i;
+    } else {
+        for (int i = 1; i <= nf; ++i) gu[i] = i;
+        for (int j = 1; j <= nf; ++j) {   // rand_perm (Trace.F90:1065-1080)
+            const int i = randint(c.rng, j, nf);
+            const int t = gu[j];
+            gu[j] = gu[i];
+            gu[i] = t;
+        }
+        gu[0] = randint(c.rng, 0, 1);
+    }
+    for (int i = 1; i <= nf; ++i) igu[gu[i]] = i;
+    igu[0] = gu[0];
+    const int old_outer = c.h.outer;
+    double sst_factor;
+    unsigned long long okm[4];
+    if (spinflip && M.has_szt) {
+        // perform_spinflip_check_qn (Trace.F90:1306-1339)
+        const int st = M.first_state[c.h.outer];
+        const int dn = st & ((1 << nb) - 1), up = (st >> nb) & ((1 << nb) - 1);
+        const int nst = up | (dn << nb);
+        c.h.outer = M.state_sst[nst];
+        global_update(c, gu);
+        sst_factor = qn_cycle_global(c, c.h.outer) ? 1.0 : 0.0;
+    } else {
+        // perform_global_check_qn (Trace.F90:1345-1383)
+        const int pold = count_valid_outer(c, okm);
+        global_update(c, gu);
+        const int pnew = count_valid_outer(c, okm);
+        if (pnew <= 0) {
+            sst_factor = 0.0;
+        } else {
+            sst_factor = (double)pnew / (double)pold;
+            int kk = randint(c.rng, 1, pnew);
+            for (int s = 0; s < M.nsst; ++s)
+                if (((okm[s / WAVE] >> (s % WAVE)) & 1ull) && --kk <= 0) {
+                    c.h.outer = s;
+                    break;
+                }
+        }
+    }
+    if (sst_factor == 0.0) {
+        c.h.outer = old_outer;
+        global_update(c, igu);
+        return c.h.qn_rej++, OUT_QN;
+    }
+    // flavour lists of the relabelled configuration in the operator scratch
+    const long kc = c.L.kcap;
+    double *tfc = c.scratch + c.L.kmat, *tfa = tfc + nf * kc;
+    int *tkf = reinterpret_cast<int *>(tfa + nf * kc);
+    rebuild_flavour_lists(c, tfc, tfa, tkf, nullptr, nullptr);
+    // det(new Delta) * det(old M) (get_InvFullHybr, get_FullLogDet); the old
+    // inverse is gathered into logical order first
+    int *piv = reinterpret_cast<int *>(c.R);
+    double dnl = 0.0, dns = 1.0, dol = 0.0, dos = 1.0;
+    const int ld = c.L.kcap;
+    bool sing = false;
+    for (int f = 0; f < nf; ++f) {
+        const int k = tkf[f];
+        if (k > 0) {
+            build_delta(c, f, k, c.scratch, tfc, tfa);
+            double l, s;
+            if (!lu_inplace(c.scratch, k, ld, l, s, piv)) sing = true;
+            dnl += l;
+            dns *= s;
+        }
+        const int ko = c.h.kf[f];
+        const int *csl = c.fcs + f * kc, *asl = c.fas + f * kc;
+        if (ko == 1) {
+            const double v = c.mi(f)[csl[0] + (long)asl[0] * ld];
+            dol += log(fabs(v));
+            dos *= copysign(1.0, v);
+        } else if (ko > 1) {
+            for (int idx = lane_id(); idx < ko * ko; idx += WAVE) {
+                const int i = idx % ko, j = idx / ko;
+                c.scratch[i + (long)j * ld] = c.mi(f)[csl[i] + (long)asl[j] * ld];
+            }
+            wsync();
+            double l, s;
+            lu_inplace(c.scratch, ko, ld, l, s, piv);
+            dol += l;
+            dos *= s;
+        }
+        const double k3 = (double)k * k * k, ko3 = (double)ko * ko * ko;
+        c.h.alg_flops += (2.0 / 3.0) * (k3 + ko3) * 2.0;
+        c.h.alg_bytes += 8.0 * ((double)k * k + (double)ko * ko);
+    }
+    double tl, ts;
+    full_trace<NM>(c, c.h.outer, tl, ts);
+    const double wr = exp((tl - c.h.trace_log) + (dnl + dol));
+    const double rr = grnd(c.rng);   // drawn even for a singular Delta (prob = 0)
+    if (!sing && rr < fabs(wr * sst_factor)) {
+        rebuild_flavour_lists(c, c.fc, c.fa, c.h.kf, c.fcs, c.fas);
+        for (int f = 0; f < nf; ++f) {
+            const int k = c.h.kf[f];
+            if (k == 0) continue;
+            build_delta(c, f, k, c.scratch, c.fc, c.fa);
+            double l, s;
+            lu_inplace(c.scratch, k, ld, l, s, piv);
+            lu_inverse(c.scratch, k, ld, piv, c.mi(f));
+        }
+        c.h.det_log = dnl;
+        c.h.det_sign = dns;
+        c.h.trace_log = tl;
+        c.h.trace_sign = ts;
+        store_part(c, c.sz[c.h.outer]);
+        c.h.cfgsign = -config_sign(c);
+        c.h.cache_pos = -1;
+        c.h.acc_glob++;
+        return OUT_ACC;
+    }
+    c.h.outer = old_outer;
+    global_update(c, igu);
+    return OUT_REJ;
+}
+
+// empty-trace weights of every superstate into c.Q (sum returned)
+template <int NM>
+DEV double empty_weights(Ctx &c)
+{
+    double sum = 0.0;
+    for (int q = 0; q < c.M.nsst; ++q) {
+        double tl, ts;
+        full_trace<NM>(c, q, tl, ts);
+        const double v = ts * exp(tl);
+        if (lane_id() == 0) c.Q[q] = v;
+        sum += v;
+    }
+    wsync();
+    return sum;
+}
+
+// choose the outer superstate proportional to the empty traces (init_Trace,
+// Trace.F90:640-670; StepChangeOuter, CTQMC.F90:4569-4637)
+template <int NM>
+DEV void choose_outer(Ctx &c)
+{
+    const double sum = empty_weights<NM>(c);
+    const double r = grnd(c.rng);
+    double acc = 0.0;
+    int pick = c.M.nsst - 1;
+    for (int q = 0; q < c.M.nsst; ++q) {
+        const double v = c.Q[q] / sum;
+        acc += v;
+        if (acc >= r && v != 0.0) {
+            pick = q;
+            break;
+        }
+    }
+    c.h.outer = pick;
+    double tl, ts;
+    full_trace<NM>(c, pick, tl, ts);
+    c.h.trace_log = tl;
+    c.h.trace_sign = ts;
+    store_part(c, c.sz[pick]);
+    c.h.cache_pos = -1;
+}
+
+// StepShiftTau_proper (CTQMC.F90:4641-4835) + update_InvFullHybrShiftTau_diag
+// (Trace.F90:2355-2457)
+template <int NM>
+DEV int step_shift(Ctx &c)
+{
+    const DevModel &M = c.M;
+    if (c.h.nop == 0) {
+        choose_outer<NM>(c);
+        c.h.cfgsign = -config_sign(c);
+        return OUT_REJ;
+    }
+    const double dtau = grnd(c.rng) * M.beta;
+    if (dtau == 0.0 || dtau == M.beta) return OUT_REJ;
+    c.h.try_shift++;
+    const int n = c.h.nop, nf = M.nf;
+    int first = lower_bound_tau(c.tau, n, dtau);
+    if (first == n) first = 0;
+    int s = c.h.outer;
+    for (int i = 0; i < first; ++i) s = c.conn(s, c.code[i], false);
+    // rotate through the operator scratch area behind the matrix scratch
+    double *opscr = c.scratch + c.L.kmat;
+    double *old_tau = opscr;
+    int *old_code = reinterpret_cast<int *>(opscr + n);
+    for (int i = lane_id(); i < n; i += WAVE) {
+        old_tau[i] = c.tau[i];
+        old_code[i] = c.code[i];
+    }
+    wsync();
+    for (int i = lane_id(); i < n; i += WAVE) {
+        const int src = (first + i) % n;
+        const double t = old_tau[src] - dtau;
+        c.tau[i] = t - floor(t / M.beta) * M.beta;
+        c.code[i] = old_code[src];
+    }
+    wsync();
+    const int old_outer = c.h.outer, old_pos = c.h.wpos;
+    const double old_min = c.h.tauwin_min, old_max = c.h.tauwin_max;
+    c.h.outer = s;
+    if (c.h.nwin > 1) {
+        const int nw = c.h.nwin;
+        const int a = (int)floor((dtau / M.beta) * (double)(nw + 1));
+        const int am = ((a % nw) + nw) % nw;
+        const int p = old_pos - am;
+        c.h.wpos = ((p % nw) + nw) % nw;
+    } else {
+        c.h.wpos = 0;
+    }
+    set_window(c);
+    double tl, ts;
+    full_trace<NM>(c, c.h.outer, tl, ts);
+    const double ratio = ts * c.h.trace_sign * exp(tl - c.h.trace_log);
+    if (grnd(c.rng) < fabs(1.0 * ratio * 1.0)) {
+        c.h.trace_log = tl;
+        c.h.trace_sign = ts;
+        store_part(c, c.sz[c.h.outer]);
+        double detrat = 1.0;
+        const int ld = c.L.kcap;
+        const long kc = c.L.kcap;
+        for (int f = 0; f < nf; ++f) {
+            const int k = c.h.kf[f];
+            double *fcf = c.fc + f * kc, *faf = c.fa + f * kc;
+            int *fcsf = c.fcs + f * kc, *fasf = c.fas + f * kc;
+            const int w1 = count_below_par(fcf, k, dtau);
+            const int w2 = count_below_par(faf, k, dtau);
+            double *Mf = c.mi(f);
+            if (!((w1 == 0 && w2 == 0) || (w1 == k && w2 == k))) {
+                // negate the physical rows of wrapped creators and columns of
+                // wrapped annihilators (entries wrapped on both sides keep sign)
+                double *rneg = c.colb, *cneg = c.rowb;
+                for (int r = lane_id(); r < k; r += WAVE) {
+                    rneg[fcsf[r]] = r < w1 ? 1.0 : 0.0;
+                    cneg[fasf[r]] = r < w2 ? 1.0 : 0.0;
+                }
+                wsync();
+                for (int idx = lane_id(); idx < k * k; idx += WAVE) {
+                    const int i = idx % k, j = idx / k;
+                    if (rneg[i] != cneg[j]) Mf[i + (long)j * ld] = -Mf[i + (long)j * ld];
+                }
+                wsync();
+            }
+            // rotate ranks: the wrapped operators move to the end, taus shifted
+            // like the operator list
+            int *rs = reinterpret_cast<int *>(c.R);
+            for (int r = lane_id(); r < k; r += WAVE) {
+                const int src = (r + w1) % k;
+                const double t = fcf[src] - dtau;
+                c.Q[r] = t - floor(t / M.beta) * M.beta;
+                rs[r] = fcsf[src];
+            }
+            wsync();
+            for (int r = lane_id(); r < k; r += WAVE) {
+                fcf[r] = c.Q[r];
+                fcsf[r] = rs[r];
+            }
+            wsync();
+            for (int r = lane_id(); r < k; r += WAVE) {
+                const int src = (r + w2) % k;
+                const double t = faf[src] - dtau;
+                c.Q[r] = t - floor(t / M.beta) * M.beta;
+                rs[r] = fasf[src];
+            }
+            wsync();
+            for (int r = lane_id(); r < k; r += WAVE) {
+                faf[r] = c.Q[r];
+                fasf[r] = rs[r];
+            }
+            wsync();
+            if (k % 2 != 0 && (w1 + w2) % 2 != 0) detrat = -detrat;
+        }
+        c.h.det_sign *= detrat;
+        c.h.cfgsign = -config_sign(c);
+        c.h.cache_pos = -1;
+        c.h.acc_shift++;
+        return OUT_ACC;
+    }
+    for (int i = lane_id(); i < n; i += WAVE) {
+        c.tau[i] = old_tau[i];
+        c.code[i] = old_code[i];
+    }
+    wsync();
+    c.h.outer = old_outer;
+    c.h.wpos = old_pos;
+    c.h.tauwin_min = old_min;
+    c.h.tauwin_max = old_max;
+    c.h.wvalid = 0;
+    return OUT_REJ;
+}
+
+// one Metropolis step with the dispatch of ctqmc_warmup / ctqmc_measure
+// (CTQMC.F90:8213-8254, 8432-8533) and the window-shift rule
+template <int NM>
+DEV void one_move(Ctx &c, int &kind, int &out)
+{
+    const DevModel &M = c.M;
+    const double r = grnd(c.rng);
+    if (r < (1.0 - M.p_shift - M.p_glob) / 2.0) {
+        kind = 1;
+        (void)grnd(c.rng);               // Percentage4OperatorMove draw
+        (void)(grnd(c.rng) < M.p_outer); // PercentageOuterMove draw
+        out = step_add<NM>(c);
+    } else if (r <= 1.0 - M.p_shift - M.p_glob) {
+        kind = 2;
+        (void)grnd(c.rng);
+        (void)(grnd(c.rng) < M.p_outer);
+        out = step_rem<NM>(c);
+    } else if (r <= 1.0 - M.p_shift) {
+        kind = 3;
+        out = step_glob<NM>(c);
+    } else {
+        kind = 4;
+        out = step_shift<NM>(c);
+    }
+    c.h.moves++;
+    if ((c.h.try_add + c.h.try_rem) % M.nslide == 0) shift_window(c);
+}
+
+// ---------------------------------------------------------------------------
+// measurements
+// ---------------------------------------------------------------------------
+// Reduce-scatter of 16 values over the 16 lanes of a DPP row: afterwards lane
+// i of the row holds the sum over the row of v[i & 15].  Pairing steps: row
+// mirror (i <-> 15-i), half-row mirror (i <-> 7-i), quad swaps (i <-> i^2,
+// i <-> i^1); at each step a lane keeps the half its position selects and
+// receives the partner's copy of that half.
+DEV double row_reduce_scatter16(double (&v)[16])
+{
+    const int i = lane_id() & 15;
+    {
+        const bool hi = i >= 8;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const double keep = hi ? v[j + 8] : v[j], give = hi ? v[j] : v[j + 8];
+            v[j] = keep + dpp_d<DPP_ROW_MIRROR>(give);
+        }
+    }
+    {
+        const bool hi = (i & 7) >= 4;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const double keep = hi ? v[j + 4] : v[j], give = hi ? v[j] : v[j + 4];
+            v[j] = keep + dpp_d<DPP_ROW_HALF_MIRROR>(give);
+        }
+    }
+    {
+        const bool hi = (i & 3) >= 2;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const double keep = hi ? v[j + 2] : v[j], give = hi ? v[j] : v[j + 2];
+            v[j] = keep + dpp_d<DPP_QP_2301>(give);
+        }
+    }
+    const bool hi = (i & 1) != 0;
+    const double keep = hi ? v[1] : v[0], give = hi ? v[0] : v[1];
+    return keep + dpp_d<DPP_QP_1032>(give);
+}
+
+// MeasGtauRem (CTQMC.F90:2252-2440): lanes over (annihilator, creator) pairs.
+// G(tau) bins by atomic adds; the Legendre sums run the Bonnet recursion per
+// lane and fold 16 orders at a time with row_reduce_scatter16 into per-lane
+// accumulators (up to 128 orders per pass over the pairs), combined across
+// rows once per flavour.  Kept out of line: inlined into the sweep kernel, the
+// ROCm 7.2 backend emits an invalid V_CMP (src_shared_base operand) for it.
+__device__ __attribute__((noinline)) void meas_gtau(Ctx &c, double *gtau, double *gleg)
+{
+    const DevModel &M = c.M;
+    const int ld = c.L.kcap;
+    constexpr int LPASS = 128, NCH = LPASS / 16;
+    for (int f = 0; f < M.nf; ++f) {
+        const int k = c.h.kf[f];
+        const double *ctl = c.fc + (long)f * c.L.kcap, *atl = c.fa + (long)f * c.L.kcap;
+        const int *csl = c.fcs + (long)f * c.L.kcap, *asl = c.fas + (long)f * c.L.kcap;
+        const double *Mf = c.mi(f);
+        c.h.alg_flops += (double)k * k * (5.0 * M.nleg + 4.0);
+        c.h.alg_bytes += 8.0 * k * k;
+        const int npair = k * k;
+        for (int l0 = 0; l0 < M.nleg; l0 += LPASS) {
+            double acc[NCH];
+#pragma unroll
+            for (int q = 0; q < NCH; ++q) acc[q] = 0.0;
+            for (int p0 = 0; p0 < npair; p0 += WAVE) {
+                const int p = p0 + lane_id();
+                double Mv = 0.0, x = 0.0;
+                if (p < npair) {
+                    const int ai = p / k, ci = p - ai * k;    // annihilator rank, creator rank
+                    const double ta = atl[ai], tc = ctl[ci];
+                    Mv = Mf[csl[ci] + (long)asl[ai] * ld];
+                    int idt;
+                    if (ta - tc > 0.0) {
+                        idt = (int)((ta - tc) / M.beta * (double)M.ntau) + 1;
+                        x = 2.0 / M.beta * (ta - tc) - 1.0;
+                    } else {
+                        idt = (int)((M.beta + ta - tc) / M.beta * (double)M.ntau) + 1;
+                        x = 2.0 / M.beta * (M.beta + ta - tc) - 1.0;
+                        Mv = -Mv;
+                    }
+                    Mv = Mv * c.h.cfgsign;
+                    if (idt > M.ntau) idt = M.ntau;
+                    if (l0 == 0) global_add(&gtau[(long)f * M.ntau + idt - 1], Mv);
+                }
+                // Bonnet recursion from order 0 up to the end of this pass
+                double pl = 0.0, plm1 = 0.0;
+                const int lend = min(M.nleg, l0 + LPASS);
+                for (int l = 0; l < l0; ++l) {
+                    const double plp1 = l == 0 ? 1.0 : (l == 1 ? x : M.legc[2 * (l - 1)] * x * pl - M.legc[2 * (l - 1) + 1] * plm1);
+                    plm1 = pl;
+                    pl = plp1;
+                }
+#pragma unroll
+                for (int q = 0; q < NCH; ++q) {
+                    const int lb = l0 + 16 * q;
+                    if (lb >= lend) break;
+                    double v[16];
+#pragma unroll
+                    for (int j = 0; j < 16; ++j) {
+                        const int l = lb + j;
+                        double plp1 = 0.0;
+                        if (l < lend) {
+                            plp1 = l == 0 ? 1.0 : (l == 1 ? x : M.legc[2 * (l - 1)] * x * pl - M.legc[2 * (l - 1) + 1] * plm1);
+                            plm1 = pl;
+                            pl = plp1;
+                        }
+                        v[j] = Mv * plp1;
+                    }
+                    acc[q] += row_reduce_scatter16(v);
+                }
+            }
+            // rows -> lanes 0..15, added to the chain's Legendre sums
+#pragma unroll
+            for (int q = 0; q < NCH; ++q) {
+                const int lb = l0 + 16 * q;
+                if (lb >= M.nleg) break;
+                double r = acc[q];
+                r += __shfl_xor(r, 16, WAVE);
+                r += __shfl_xor(r, 32, WAVE);
+                const int l = lb + lane_id();
+                if (lane_id() < 16 && l < M.nleg) gleg[(long)f * M.nleg + l] += r;
+            }
+        }
+    }
+    wsync();
+}
+
+// MeasDensityMatrix_beta_half (CTQMC.F90:3088-3151) with update_b2_states_eb
+// (Trace.F90:6213-6469): ket from 0 to beta/2, bra from beta to beta/2
+template <int NM>
+DEV void meas_dm(Ctx &c, double *dm)
+{
+    const DevModel &M = c.M;
+    const int outer = c.h.outer, no = c.sz[outer];
+    if (c.h.nop != 0) {
+        const double bh = M.beta / 2.0;
+        const int nk = upper_bound_tau(c.tau, c.h.nop, bh);   // ops with tau <= beta/2
+        double *K = c.K0, *Kw = c.K1;
+        set_identity(K, no);
+        int s = outer;
+        double kl = 0.0;
+        ket_walk_global<NM>(c, K, Kw, s, kl, no, 0.0, 0, nk, bh);
+        double *Ks = c.scratch;
+        copy_mat(Ks, K, c.sz[s] * no);
+        double *B = c.K0, *Bw = c.K1;
+        set_identity(B, no);
+        int sb = outer;
+        double bl = 0.0;
+        bra_walk_global<NM>(c, B, Bw, sb, bl, no, M.beta, nk, c.h.nop, bh);
+        const int n = c.sz[s];
+        double *blk = dm + M.dmoff[s];
+        const double fac = c.h.trace_sign * (-c.h.cfgsign) * exp(kl + bl - c.h.trace_log);
+        for (int idx = lane_id(); idx < n * n; idx += WAVE) {
+            const int s1 = idx % n, s2 = idx / n;
+            double acc = 0.0;
+            for (int i = 0; i < no; ++i) acc = fma(B[s1 * no + i], Ks[s2 * no + i], acc);
+            blk[s1 + (long)s2 * n] += fac * acc;
+        }
+        wsync();
+    } else {
+        double *blk = dm + M.dmoff[outer];
+        for (int i = lane_id(); i < no; i += WAVE) {
+            const double v = c.part[2 * i + 1] * c.h.trace_sign * exp(c.part[2 * i] - c.h.trace_log);
+            blk[i + (long)i * no] -= c.h.cfgsign * v;
+        }
+        wsync();
+    }
+}
+
+// MeasSign, MeasOuterHistograms (visits), MeasExpOrder (CTQMC.F90:3264-3350)
+DEV void meas_misc(Ctx &c, double *histo, double *outer_hist)
+{
+    const DevModel &M = c.M;
+    c.h.meas_sign += c.h.cfgsign;
+    if (lane_id() == 0) {
+        outer_hist[c.h.outer] += 1.0;
+        for (int f = 0; f < M.nf; ++f) {
+            int k = c.h.kf[f];
+            if (k > M.maxhisto) k = M.maxhisto;
+            histo[(long)f * (M.maxhisto + 1) + k] += 1.0;
+        }
+    }
+    wsync();
+}
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+// Map the chain block and carve the dynamic LDS; copies header, RNG state and
+// the model tables into LDS.
+template <int NM>
+DEV Ctx make_ctx(const DevModel &M, const ChainLayout &L, char *base, char *lds)
+{
+    ChainHdr &hd = *reinterpret_cast<ChainHdr *>(lds + L.l_hdr);
+    Ctx c(M, L, base, hd);
+    c.nm = NM;
+    c.tau = reinterpret_cast<double *>(base + L.tau);
+    c.code = reinterpret_cast<int *>(base + L.code);
+    c.fc = reinterpret_cast<double *>(base + L.fc);
+    c.fa = reinterpret_cast<double *>(base + L.fa);
+    c.fcs = reinterpret_cast<int *>(base + L.fcs);
+    c.fas = reinterpret_cast<int *>(base + L.fas);
+    c.minv = reinterpret_cast<double *>(base + L.minv);
+    c.scratch = reinterpret_cast<double *>(base + L.scratch);
+    c.kst = reinterpret_cast<double *>(base + L.kst);
+    c.bnd = reinterpret_cast<double *>(base + L.bnd);
+    c.bndlog = reinterpret_cast<double *>(base + L.bndlog);
+    c.bndsst = reinterpret_cast<int *>(base + L.bndsst);
+    c.part = reinterpret_cast<double *>(base + L.part);
+    uint32_t *mt = reinterpret_cast<uint32_t *>(lds + L.l_mt);
+    int *sz = reinterpret_cast<int *>(lds + L.l_sz);
+    int *off = reinterpret_cast<int *>(lds + L.l_off);
+    int *cn = reinterpret_cast<int *>(lds + L.l_cn);
+    int *oo = reinterpret_cast<int *>(lds + L.l_oo);
+    double *ev = reinterpret_cast<double *>(lds + L.l_ev);
+    c.wtau = reinterpret_cast<double *>(lds + L.l_wtau);
+    c.wcode = reinterpret_cast<int *>(lds + L.l_wcode);
+    c.dvec = reinterpret_cast<double *>(lds + L.l_dvec);
+    c.Q = reinterpret_cast<double *>(lds + L.l_q);
+    c.R = reinterpret_cast<double *>(lds + L.l_r);
+    c.colb = reinterpret_cast<double *>(lds + L.l_col);
+    c.rowb = reinterpret_cast<double *>(lds + L.l_row);
+    c.pl = reinterpret_cast<double *>(lds + L.l_pl);
+    if (L.l_k >= 0) {
+        c.K0 = reinterpret_cast<double *>(lds + L.l_k);
+        c.K1 = c.K0 + (long)NM * NM;
+    } else {
+        c.K0 = reinterpret_cast<double *>(base + L.kwork);
+        c.K1 = c.K0 + (long)NM * NM;
+    }
+    // header, RNG state and model tables into LDS
+    const int *gh = reinterpret_cast<const int *>(base);
+    int *lh = reinterpret_cast<int *>(&hd);
+    for (int i = lane_id(); i < (int)(sizeof(ChainHdr) / 4); i += WAVE) lh[i] = gh[i];
+    const uint32_t *gmt = reinterpret_cast<const uint32_t *>(base + L.mt);
+    for (int i = lane_id(); i < 624; i += WAVE) mt[i] = gmt[i];
+    for (int i = lane_id(); i < M.nsst; i += WAVE) {
+        sz[i] = M.sizes[i];
+        off[i] = M.offsets[i];
+    }
+    const int ncn = M.nsst * M.nb * 4;
+    for (int i = lane_id(); i < ncn; i += WAVE) {
+        cn[i] = M.connect[i];
+        oo[i] = (int)M.opoff[i];
+    }
+    for (int i = lane_id(); i < M.nstates; i += WAVE) ev[i] = M.evals[i];
+    if (L.l_ops >= 0) {
+        double *lops = reinterpret_cast<double *>(lds + L.l_ops);
+        for (long i = lane_id(); i < M.nops; i += WAVE) lops[i] = M.ops[i];
+        c.opsp = lops;
+    } else {
+        c.opsp = M.ops;
+    }
+    c.sz = sz, c.off = off, c.cn = cn, c.oo = oo, c.ev = ev;
+    c.rng.mt = mt;
+    c.rng.idx = &hd.mti;
+    wsync();
+    return c;
+}
+
+DEV void save_ctx(Ctx &c)
+{
+    uint32_t *gmt = reinterpret_cast<uint32_t *>(c.base + c.L.mt);
+    for (int i = lane_id(); i < 624; i += WAVE) gmt[i] = c.rng.mt[i];
+    int *gh = reinterpret_cast<int *>(c.base);
+    const int *lh = reinterpret_cast<const int *>(&c.h);
+    wsync();
+    for (int i = lane_id(); i < (int)(sizeof(ChainHdr) / 4); i += WAVE) gh[i] = lh[i];
+    wsync();
+}
+
+// Sweep kernel: each block advances chain blockIdx.x by `nmoves` steps; if
+// ncorr > 0 it measures after every ncorr steps.  rec != nullptr records
+// every step of chain `only` (parity mode; other chains return at once).
+template <int NM>
+__global__ void __launch_bounds__(WAVE) sweep_kernel(DevModel M, ChainLayout L, char *chains, long long nmoves,
+                                                     long long ncorr, int only, w2d_move_record *rec)
+{
+    const int chain = blockIdx.x;
+    if (only >= 0 && chain != only) return;
+    extern __shared__ __align__(16) char lds[];
+    char *base = chains + (long)chain * L.stride;
+    Ctx c = make_ctx<NM>(M, L, base, lds);
+    double *gtau = reinterpret_cast<double *>(base + L.gtau);
+    double *gleg = reinterpret_cast<double *>(base + L.gleg);
+    double *dm = reinterpret_cast<double *>(base + L.dm);
+    double *histo = reinterpret_cast<double *>(base + L.histo);
+    double *ohist = reinterpret_cast<double *>(base + L.outer);
+    for (long long m = 0; m < nmoves && c.h.err == 0; ++m) {
+        int kind, out;
+        one_move<NM>(c, kind, out);
+        if (rec && lane_id() == 0) {
+            w2d_move_record &r = rec[m];
+            r.kind = kind;
+            r.outcome = out;
+            r.noper = c.h.nop;
+            r.outer_sst = c.h.outer;
+            r.window = c.h.wpos;
+            r.trace_log = c.h.trace_log;
+            r.trace_sign = c.h.trace_sign;
+            r.det_log = c.h.det_log;
+            r.det_sign = c.h.det_sign;
+            r.cfgsign = c.h.cfgsign;
+        }
+        if (ncorr > 0 && (m + 1) % ncorr == 0) {
+            c.h.cnt_meas++;
+            meas_misc(c, histo, ohist);
+            if (M.meas_dm) meas_dm<NM>(c, dm);
+            meas_gtau(c, gtau, gleg);
+        }
+    }
+    save_ctx(c);
+}
+
+// Initial state of every chain: empty configuration, initial outer superstate
+// drawn proportional to the empty traces (init_Trace, Trace.F90:640-670).
+template <int NM>
+__global__ void __launch_bounds__(WAVE) init_kernel(DevModel M, ChainLayout L, char *chains)
+{
+    extern __shared__ __align__(16) char lds[];
+    char *base = chains + (long)blockIdx.x * L.stride;
+    Ctx c = make_ctx<NM>(M, L, base, lds);
+    choose_outer<NM>(c);
+    if (!(c.h.trace_log > -DBL_MAX)) c.h.err = 5;
+    save_ctx(c);
+}
+
+// Adopt the configuration already written into tau/code/nop/outer: full trace,
+// flavour lists, full inverses, sign (set_mc_config_from_arrays,
+// Trace.F90:6872-7000).
+template <int NM>
+__global__ void __launch_bounds__(WAVE) setconfig_kernel(DevModel M, ChainLayout L, char *chains, int only)
+{
+    if ((int)blockIdx.x != only) return;
+    extern __shared__ __align__(16) char lds[];
+    char *base = chains + (long)blockIdx.x * L.stride;
+    Ctx c = make_ctx<NM>(M, L, base, lds);
+    c.h.wpos = 0;
+    set_window(c);
+    double tl, ts;
+    full_trace<NM>(c, c.h.outer, tl, ts);
+    c.h.trace_log = tl;
+    c.h.trace_sign = ts;
+    store_part(c, c.sz[c.h.outer]);
+    rebuild_flavour_lists(c, c.fc, c.fa, c.h.kf, c.fcs, c.fas);
+    int *piv = reinterpret_cast<int *>(c.R);
+    double dl = 0.0, dsg = 1.0;
+    for (int f = 0; f < M.nf; ++f) {
+        const int k = c.h.kf[f];
+        if (k == 0) continue;
+        build_delta(c, f, k, c.scratch, c.fc, c.fa);
+        double l, sg;
+        if (!lu_inplace(c.scratch, k, L.kcap, l, sg, piv)) {
+            c.h.err = ERR_SING;
+            break;
+        }
+        lu_inverse(c.scratch, k, L.kcap, piv, c.mi(f));
+        dl += l;
+        dsg *= sg;
+    }
+    c.h.det_log = dl;
+    c.h.det_sign = dsg;
+    c.h.cfgsign = -config_sign(c);
+    c.h.cache_pos = -1;
+    save_ctx(c);
+}
+
+}  // namespace w2d

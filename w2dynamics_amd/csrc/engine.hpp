@@ -16,7 +16,9 @@ struct DevModel {
     const long *opoff;
     const double *ops, *evals, *ftau;   // ftau[f][nftau]
     const long *dmoff;                  // block-diagonal density-matrix offsets
+    const double *legc;                 // Bonnet coefficients (2l+1)/(l+1), l/(l+1) per l
     long dmsize;
+    long nops;                          // size of ops
     double p_shift, p_glob, p_outer, p_4op;
     long long nslide;
     int ntau, nleg, maxhisto, has_szt, meas_dm;
@@ -26,6 +28,7 @@ struct DevModel {
 struct ChainHdr {
     int nop, outer, wpos, nwin;
     int err, cache_pos, kst_sst, mti;
+    int wia, wib, wvalid, pad0;    // cached index range of the window's operators
     double taudiff_max, tauwin_min, tauwin_max, kst_log;
     double trace_log, trace_sign, det_log, det_sign, cfgsign;
     double meas_sign;
@@ -43,6 +46,7 @@ struct ChainLayout {
     long mt;              // uint32[624]
     long tau, code;       // double[cap], int[cap]: configuration sorted by tau
     long fc, fa;          // double[nf][kcap]: creator / annihilator taus per flavour
+    long fcs, fas;        // int[nf][kcap]: physical row / column slot of each rank
     long minv;            // double[nf][kcap*kcap]: inverse hybridisation, column-major
     long scratch;         // double[kmat + 4*cap]: matrix scratch, then operator scratch
     long kst;             // double[nmax*nmax]: ket at the window start
@@ -54,6 +58,10 @@ struct ChainLayout {
     long gtau, gleg, dm, histo, outer; // accumulators
     int cap, kcap, nbound, lcap;
     long kmat;            // max(kcap*kcap, nmax*nmax)
+    // dynamic LDS carve-up (byte offsets; l_ops / l_k = -1: kept in global memory)
+    long l_hdr, l_mt, l_sz, l_off, l_cn, l_oo, l_ev, l_ops, l_wtau, l_wcode;
+    long l_dvec, l_q, l_r, l_col, l_row, l_pl, l_k, lds_bytes;
+    int lcap_lds;         // operators the LDS list holds (window + 2)
 };
 
 }  // namespace w2d

@@ -1,0 +1,467 @@
+!===============================================================================
+!> Fortran host of the MI355X CT-HYB sampling core.
+!!
+!! Module MCTQMC keeps the entry points, argument meanings and result arrays
+!! that the w2dyn Python driver uses from the reference solver module
+!! (w2dynamics src/ctqmc_fortran/CTQMC.F90; f2py list in
+!! src/ctqmc_fortran/CMakeLists.txt:61-72, calls in
+!! w2dyn/dmft/cthyb_solver.py:240-342 and reads in :949-1030).  Every call goes
+!! through the C-ABI of libw2dyn_ctqmc.so (include/w2dyn_ctqmc.h) by
+!! iso_c_binding; no sampling happens here.
+!!
+!! Differences from the reference module, all on purpose:
+!!  - one handle runs n_chains independent Markov chains on one GPU (chain c
+!!    seeded with Nseed + c, like MPI ranks NSeed + rank); the result arrays are
+!!    the mean over chains of each chain's normalised observables, i.e. what the
+!!    driver obtains from the mean over ranks.  n_chains = 1 (default) is the
+!!    reference's single-rank run move for move;
+!!  - errors are returned by the library as status codes; this layer prints
+!!    the code and sets `aborted` = 2 instead of stopping the process;
+!!  - only the Z-space sector (iSector = 1) with diagonal hybridisation is
+!!    available; other sectors set aborted = 2.
+!===============================================================================
+module MCTQMC
+   use iso_c_binding
+   implicit none
+
+   integer, parameter :: KINDR = kind(0.d0)
+
+   !> batching controls (no counterpart in the reference module)
+   integer :: n_chains = 1
+   integer :: chain_capacity = 256
+   integer :: device_id = 0
+   integer :: config_chain = 0
+
+   !> reference module variables used by the driver
+   integer :: SimID = -1
+   integer :: aborted = 0
+   logical :: fancy_prog = .false.
+   real(KINDR) :: time_warmup = 0, time_sim = 0, time_sim_steps = 0
+   real(KINDR) :: mean_sign = 0
+   real(KINDR), allocatable :: Gtau(:,:,:)           ! (NBands,2,Ntau)
+   real(KINDR), allocatable :: GLeg(:,:,:)           ! (NBands,2,NLegMax)
+   real(KINDR), allocatable :: occ(:,:,:,:)          ! (NBands,2,NBands,2)
+   real(KINDR), allocatable :: DensityMatrix(:,:)    ! (NStates,NStates), occupation basis
+   real(KINDR), allocatable :: Histo(:,:,:)          ! (NBands,2,0:MaxHisto)
+   real(KINDR), allocatable :: OuterSuperstateHisto(:)
+   real(KINDR), allocatable :: EigenstatesEnergies(:)
+   integer, allocatable :: StatesSuperstates(:)      ! superstate of each eigen index
+   integer, allocatable :: OccbasisMapping(:)        ! occupation-basis state of each index
+   real(KINDR) :: AccAdd(1) = 0, AccRem(1) = 0, AccGlob = 0, AccShift = 0
+   integer(c_int64_t) :: TryAdd(1) = 0, TryRem(1) = 0, TryGlob = 0, TryShift = 0
+
+   type(c_ptr), private, save :: handle = c_null_ptr
+   character(len=:), allocatable, private, save :: paramstring
+   integer, private, save :: seed = 0
+   integer, private, save :: nb = 0, ntau = 0, nleg = 0, nsst = 0, nstates = 0, maxhisto = 0
+   real(KINDR), private, save :: beta = 0
+   integer(c_int), allocatable, private, save :: model_sizes(:), model_offs(:)
+   real(KINDR), allocatable, private, save :: model_vec(:)
+
+   interface
+      integer(c_int) function w2d_ctqmc_create(params, nbands, nftau, u, ftau, muimp, nchains, &
+                                               seed0, capacity, device, out) &
+            bind(C, name="w2d_ctqmc_create")
+         import :: c_int, c_char, c_double, c_int32_t, c_ptr
+         character(kind=c_char), dimension(*), intent(in) :: params
+         integer(c_int), value :: nbands, nftau
+         real(c_double), dimension(*), intent(in) :: u, ftau, muimp
+         integer(c_int), value :: nchains
+         integer(c_int32_t), value :: seed0
+         integer(c_int), value :: capacity, device
+         type(c_ptr), intent(out) :: out
+      end function
+      integer(c_int) function w2d_ctqmc_set_taudiffmax(h, x) bind(C, name="w2d_ctqmc_set_taudiffmax")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: h
+         real(c_double), value :: x
+      end function
+      integer(c_int) function w2d_ctqmc_warmup(h, n) bind(C, name="w2d_ctqmc_warmup")
+         import :: c_int, c_ptr, c_int64_t
+         type(c_ptr), value :: h
+         integer(c_int64_t), value :: n
+      end function
+      integer(c_int) function w2d_ctqmc_measure(h, n) bind(C, name="w2d_ctqmc_measure")
+         import :: c_int, c_ptr, c_int64_t
+         type(c_ptr), value :: h
+         integer(c_int64_t), value :: n
+      end function
+      integer(c_int) function w2d_ctqmc_dims(h, nbands, nchains, ntau, nlegmax, nsst, nstates, &
+                                             maxhisto) bind(C, name="w2d_ctqmc_dims")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: h
+         integer(c_int), intent(out) :: nbands, nchains, ntau, nlegmax, nsst, nstates, maxhisto
+      end function
+      integer(c_int) function w2d_ctqmc_get_accumulators(h, gtau, gleg, dm, histo, outer, scalars) &
+            bind(C, name="w2d_ctqmc_get_accumulators")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: h
+         real(c_double), dimension(*), intent(out) :: gtau, gleg, dm, histo, outer, scalars
+      end function
+      integer(c_int) function w2d_ctqmc_get_model(h, sizes, offsets, evals, states, evecs, egs) &
+            bind(C, name="w2d_ctqmc_get_model")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: h
+         type(c_ptr), value :: sizes, offsets, evals, states, evecs, egs   ! NULL = not requested
+      end function
+      integer(c_int) function w2d_ctqmc_get_config(h, chain, cap, n, tau, orb, spin, ca, outer_sst, &
+                                                   weights) bind(C, name="w2d_ctqmc_get_config")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: h
+         integer(c_int), value :: chain, cap
+         integer(c_int), intent(out) :: n, outer_sst
+         real(c_double), dimension(*), intent(out) :: tau, weights
+         integer(c_int), dimension(*), intent(out) :: orb, spin, ca
+      end function
+      integer(c_int) function w2d_ctqmc_set_config(h, chain, n, tau, orb, spin, ca, outer_sst, &
+                                                   weights) bind(C, name="w2d_ctqmc_set_config")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: h
+         integer(c_int), value :: chain, n, outer_sst
+         real(c_double), dimension(*), intent(in) :: tau
+         integer(c_int), dimension(*), intent(in) :: orb, spin, ca
+         real(c_double), dimension(*), intent(out) :: weights
+      end function
+      integer(c_int) function w2d_ctqmc_destroy(h) bind(C, name="w2d_ctqmc_destroy")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: h
+      end function
+   end interface
+
+contains
+
+   !> status check: print and flag instead of stopping (the reference stops)
+   logical function failed(st, where)
+      integer(c_int), intent(in) :: st
+      character(*), intent(in) :: where
+      failed = st /= 0
+      if (failed) then
+         write (0, '(A,A,A,I0)') "MCTQMC: ", where, " returned status ", st
+         aborted = 2
+      end if
+   end function failed
+
+   !> value of `key` in the parameter string (first line whose key starts with it)
+   function param_value(key) result(val)
+      character(*), intent(in) :: key
+      character(len=:), allocatable :: val
+      integer :: i0, i1, ieq, ih
+      character(len=:), allocatable :: line
+      val = ""
+      i0 = 1
+      do while (i0 <= len(paramstring))
+         i1 = index(paramstring(i0:), new_line('a'))
+         if (i1 == 0) then
+            line = paramstring(i0:)
+            i0 = len(paramstring) + 1
+         else
+            line = paramstring(i0:i0 + i1 - 2)
+            i0 = i0 + i1
+         end if
+         ieq = index(line, "=")
+         if (ieq == 0) cycle
+         if (index(adjustl(line(:ieq - 1)), key) /= 1) cycle
+         ih = index(line(ieq + 1:), "#")
+         if (ih == 0) then
+            val = trim(adjustl(line(ieq + 1:)))
+         else
+            val = trim(adjustl(line(ieq + 1:ieq + ih - 1)))
+         end if
+         return
+      end do
+   end function param_value
+
+   !> init_paras (CTQMC.F90:9412): the "key = value" parameter string
+   subroutine init_paras(parastr)
+      character(*), intent(in) :: parastr
+      paramstring = parastr
+   end subroutine init_paras
+
+   !> dest_paras (CTQMC.F90:9420)
+   subroutine dest_paras()
+      if (allocated(paramstring)) deallocate (paramstring)
+   end subroutine dest_paras
+
+   !> init_rnd (CTQMC.F90:9404): chain c is seeded with Nseed + c
+   subroutine init_rnd(Nseed)
+      integer, intent(in) :: Nseed
+      seed = Nseed
+   end subroutine init_rnd
+
+   !> init_solver (CTQMC.F90:7894-8085): local model, hybridisation and chains.
+   !! screening_function and iter_no are accepted for the same call shape;
+   !! retarded interactions are not supported by the library.
+   subroutine init_solver(u_matrix_in, Ftau_full, muimp_full, screening_function, iter_no, Nftau, NBands)
+      integer, intent(in) :: iter_no, Nftau, NBands
+      real(KINDR), intent(in) :: u_matrix_in(2*NBands, 2*NBands, 2*NBands, 2*NBands)
+      real(KINDR), intent(in) :: Ftau_full(NBands, 2, NBands, 2, Nftau)
+      real(KINDR), intent(in) :: muimp_full(NBands, 2, NBands, 2)
+      real(KINDR), intent(in) :: screening_function(NBands, 2, NBands, 2, Nftau)
+      integer(c_int) :: st, nbr, ncr
+      character(len=:), allocatable :: bs
+      if (.not. allocated(paramstring)) then
+         write (0, '(A)') "MCTQMC: init_paras must precede init_solver"
+         aborted = 2
+         return
+      end if
+      if (c_associated(handle)) call dest_ctqmc()
+      aborted = 0
+      st = w2d_ctqmc_create(paramstring//c_null_char, int(NBands, c_int), int(Nftau, c_int), &
+                            u_matrix_in, Ftau_full, muimp_full, int(n_chains, c_int), &
+                            int(seed, c_int32_t), int(chain_capacity, c_int), int(device_id, c_int), handle)
+      if (failed(st, "w2d_ctqmc_create")) then
+         handle = c_null_ptr
+         return
+      end if
+      st = w2d_ctqmc_dims(handle, nbr, ncr, ntau, nleg, nsst, nstates, maxhisto)
+      if (failed(st, "w2d_ctqmc_dims")) return
+      nb = nbr
+      bs = param_value("beta")
+      read (bs, *) beta
+      call load_model()
+      if (.false.) print *, iter_no, screening_function(1, 1, 1, 1, 1)
+   end subroutine init_solver
+
+   subroutine load_model()
+      integer(c_int), allocatable, target :: sizes(:), offs(:), states(:)
+      real(KINDR), allocatable, target :: ev(:), vec(:)
+      real(KINDR), target :: egs(1)
+      integer(c_int) :: st
+      integer :: s, i
+      allocate (sizes(nsst), offs(nsst), states(nstates), ev(nstates))
+      ! block sizes first, then the eigenvector blocks (column-major, concatenated)
+      st = w2d_ctqmc_get_model(handle, c_loc(sizes), c_loc(offs), c_loc(ev), c_loc(states), &
+                               c_null_ptr, c_loc(egs))
+      if (failed(st, "w2d_ctqmc_get_model")) return
+      allocate (vec(max(1_8, sum(int(sizes, 8)**2))))
+      st = w2d_ctqmc_get_model(handle, c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, c_loc(vec), &
+                               c_null_ptr)
+      if (failed(st, "w2d_ctqmc_get_model")) return
+      EigenstatesEnergies = ev
+      OccbasisMapping = states
+      if (allocated(StatesSuperstates)) deallocate (StatesSuperstates)
+      allocate (StatesSuperstates(nstates))
+      do s = 1, nsst
+         do i = 1, sizes(s)
+            StatesSuperstates(offs(s) + i) = s - 1
+         end do
+      end do
+      model_sizes = sizes
+      model_offs = offs
+      model_vec = vec
+   end subroutine load_model
+
+   !> set_taudiffmax (CTQMC.F90:8180)
+   subroutine set_taudiffmax(taudiffmax)
+      real(KINDR), intent(in) :: taudiffmax
+      if (failed(w2d_ctqmc_set_taudiffmax(handle, taudiffmax), "w2d_ctqmc_set_taudiffmax")) return
+   end subroutine set_taudiffmax
+
+   !> init_counters (CTQMC.F90:847): the library clears its accumulators at the
+   !! start of every measurement call
+   subroutine init_counters()
+   end subroutine init_counters
+
+   !> ctqmc_warmup (CTQMC.F90:8188): Nwarmups steps of every chain
+   subroutine ctqmc_warmup(Nwarmups)
+      integer(c_int64_t), intent(in) :: Nwarmups
+      integer(8) :: t0, t1, rate
+      call system_clock(t0, rate)
+      if (failed(w2d_ctqmc_warmup(handle, Nwarmups), "w2d_ctqmc_warmup")) return
+      call system_clock(t1)
+      time_warmup = real(t1 - t0, KINDR)/real(rate, KINDR)
+   end subroutine ctqmc_warmup
+
+   !> ctqmc_measure (CTQMC.F90:8355-8771), Z-space sector: Nmeas x (NCorr steps +
+   !! measurements) per chain, then the normalisation of CTQMC.F90:8663-8771
+   !! per chain and the mean over chains into the module arrays.
+   subroutine ctqmc_measure(iSector, iComponent)
+      integer, intent(in) :: iSector, iComponent
+      integer(c_int64_t) :: nmeas
+      integer(8) :: t0, t1, rate
+      character(len=:), allocatable :: v
+      if (iSector /= 1) then
+         write (0, '(A,I0,A,I0)') "MCTQMC: only the Z-space sector is available, got sector ", &
+            iSector, " component ", iComponent
+         aborted = 2
+         return
+      end if
+      v = param_value("Nmeas")
+      read (v, *) nmeas
+      call system_clock(t0, rate)
+      if (failed(w2d_ctqmc_measure(handle, nmeas), "w2d_ctqmc_measure")) return
+      call system_clock(t1)
+      time_sim = real(t1 - t0, KINDR)/real(rate, KINDR)
+      time_sim_steps = time_sim
+      call collect_results()
+   end subroutine ctqmc_measure
+
+   subroutine collect_results()
+      real(KINDR), allocatable :: g(:,:,:), gl(:,:,:), dm(:,:,:), hi(:,:,:), oh(:,:), sc(:,:)
+      real(KINDR), allocatable :: dmb(:,:), v(:,:), diag(:)
+      real(KINDR) :: cnt, msign, zs, lfac
+      integer :: c, b, s, f, l, k, o, n, b2, s2, voff
+      integer(c_int) :: st
+      allocate (g(ntau, 2*nb, n_chains), gl(nleg, 2*nb, n_chains), dm(nstates, nstates, n_chains))
+      allocate (hi(maxhisto + 1, 2*nb, n_chains), oh(nsst, n_chains), sc(14, n_chains))
+      st = w2d_ctqmc_get_accumulators(handle, g, gl, dm, hi, oh, sc)
+      if (failed(st, "w2d_ctqmc_get_accumulators")) return
+      call reset_results()
+      allocate (diag(nstates))
+      do c = 1, n_chains
+         cnt = sc(2, c)
+         msign = sc(1, c)/cnt
+         zs = -1.0_KINDR/(cnt*msign)
+         mean_sign = mean_sign + msign/n_chains
+         do s = 1, 2
+            do b = 1, nb
+               f = b + nb*(s - 1)
+               Gtau(b, s, :) = Gtau(b, s, :) - (g(:, f, c)/cnt)/(-msign)*ntau/beta**2/n_chains
+               do l = 1, nleg
+                  lfac = sqrt(2.0_KINDR*(l - 1) + 1.0_KINDR)/beta
+                  GLeg(b, s, l) = GLeg(b, s, l) - gl(l, f, c)*lfac*zs/n_chains
+               end do
+               Histo(b, s, :) = Histo(b, s, :) + hi(:, f, c)/cnt/n_chains
+            end do
+         end do
+         OuterSuperstateHisto = OuterSuperstateHisto + oh(:, c)/cnt/n_chains
+         ! density matrix: eigenbasis blocks -> occupation basis (V rho V^T)
+         voff = 0
+         do k = 1, nsst
+            o = model_offs(k)
+            n = model_sizes(k)
+            v = reshape(model_vec(voff + 1:voff + n*n), [n, n])
+            dmb = dm(o + 1:o + n, o + 1:o + n, c)*zs
+            DensityMatrix(o + 1:o + n, o + 1:o + n) = DensityMatrix(o + 1:o + n, o + 1:o + n) + &
+                                                      matmul(v, matmul(dmb, transpose(v)))/n_chains
+            voff = voff + n*n
+         end do
+         AccAdd(1) = AccAdd(1) + sc(3, c)/max(sc(4, c), 1.0_KINDR)/n_chains
+         AccRem(1) = AccRem(1) + sc(5, c)/max(sc(6, c), 1.0_KINDR)/n_chains
+         AccGlob = AccGlob + sc(7, c)/max(sc(8, c), 1.0_KINDR)/n_chains
+         AccShift = AccShift + sc(9, c)/max(sc(10, c), 1.0_KINDR)/n_chains
+         TryAdd(1) = TryAdd(1) + int(sc(4, c), c_int64_t)
+         TryRem(1) = TryRem(1) + int(sc(6, c), c_int64_t)
+         TryGlob = TryGlob + int(sc(8, c), c_int64_t)
+         TryShift = TryShift + int(sc(10, c), c_int64_t)
+      end do
+      ! occ(b1,s1,b2,s2) = sum_k rho_kk n_{b1 s1}(k) n_{b2 s2}(k), occupation basis
+      do k = 1, nstates
+         diag(k) = DensityMatrix(k, k)
+      end do
+      do s2 = 1, 2
+         do b2 = 1, nb
+            do s = 1, 2
+               do b = 1, nb
+                  do k = 1, nstates
+                     if (btest(OccbasisMapping(k), b - 1 + nb*(s - 1)) .and. &
+                         btest(OccbasisMapping(k), b2 - 1 + nb*(s2 - 1))) &
+                        occ(b, s, b2, s2) = occ(b, s, b2, s2) + diag(k)
+                  end do
+               end do
+            end do
+         end do
+      end do
+   end subroutine collect_results
+
+   subroutine reset_results()
+      if (allocated(Gtau)) deallocate (Gtau, GLeg, occ, DensityMatrix, Histo, OuterSuperstateHisto)
+      allocate (Gtau(nb, 2, ntau), GLeg(nb, 2, nleg), occ(nb, 2, nb, 2), DensityMatrix(nstates, nstates))
+      allocate (Histo(nb, 2, 0:maxhisto), OuterSuperstateHisto(nsst))
+      Gtau = 0
+      GLeg = 0
+      occ = 0
+      DensityMatrix = 0
+      Histo = 0
+      OuterSuperstateHisto = 0
+      mean_sign = 0
+      AccAdd = 0
+      AccRem = 0
+      AccGlob = 0
+      AccShift = 0
+      TryAdd = 0
+      TryRem = 0
+      TryGlob = 0
+      TryShift = 0
+   end subroutine reset_results
+
+   !> get_mc_config_scalars (CTQMC.F90:9358) of chain config_chain
+   subroutine get_mc_config_scalars(N, outer_sst, outer_state)
+      integer, intent(out) :: N, outer_sst, outer_state
+      integer(c_int) :: nn, osst
+      real(KINDR) :: tau(1), w(5)
+      integer(c_int) :: idum(1)
+      N = 0
+      outer_sst = 0
+      outer_state = 0
+      if (failed(w2d_ctqmc_get_config(handle, int(config_chain, c_int), 0_c_int, nn, tau, idum, idum, &
+                                      idum, osst, w), "w2d_ctqmc_get_config")) return
+      N = nn
+      outer_sst = osst
+   end subroutine get_mc_config_scalars
+
+   !> get_mc_config (CTQMC.F90:9366): orbitals and spins 1-based, ca 1 = creator
+   subroutine get_mc_config(N, taus, orbs, spins, cas, hashybs, outer_sst, outer_state)
+      integer, intent(in) :: N
+      real(KINDR), dimension(N), intent(out) :: taus
+      integer, dimension(N), intent(out) :: orbs, spins, cas, hashybs
+      integer, intent(out) :: outer_sst, outer_state
+      integer(c_int) :: nn, osst
+      integer(c_int) :: o(max(N, 1)), sp(max(N, 1)), ca(max(N, 1))
+      real(KINDR) :: t(max(N, 1)), w(5)
+      outer_state = 0
+      outer_sst = 0
+      if (failed(w2d_ctqmc_get_config(handle, int(config_chain, c_int), int(N, c_int), nn, t, o, sp, &
+                                      ca, osst, w), "w2d_ctqmc_get_config")) return
+      if (nn /= N) then
+         aborted = 2
+         return
+      end if
+      taus = t(1:N)
+      orbs = o(1:N) + 1
+      spins = sp(1:N) + 1
+      cas = ca(1:N)
+      hashybs = -1
+      outer_sst = osst
+   end subroutine get_mc_config
+
+   !> set_mc_config (CTQMC.F90:9378) for chain config_chain
+   subroutine set_mc_config(N, taus, orbs, spins, cas, hashybs, outer_sst, outer_state)
+      integer, intent(in) :: N
+      real(KINDR), dimension(N), intent(in) :: taus
+      integer, dimension(N), intent(in) :: orbs, spins, cas, hashybs
+      integer, intent(in) :: outer_sst, outer_state
+      integer(c_int) :: o(max(N, 1)), sp(max(N, 1)), ca(max(N, 1))
+      real(KINDR) :: t(max(N, 1)), w(5)
+      if (any(hashybs /= -1)) then
+         write (0, '(A)') "MCTQMC: set_mc_config supports hybridised operators only"
+         aborted = 2
+         return
+      end if
+      t(1:N) = taus
+      o(1:N) = orbs - 1
+      sp(1:N) = spins - 1
+      ca(1:N) = cas
+      if (failed(w2d_ctqmc_set_config(handle, int(config_chain, c_int), int(N, c_int), t, o, sp, ca, &
+                                      int(outer_sst, c_int), w), "w2d_ctqmc_set_config")) return
+      if (.false.) print *, outer_state
+   end subroutine set_mc_config
+
+   !> set_empty_mc_config (CTQMC.F90:9389)
+   subroutine set_empty_mc_config(outer_sst, outer_state)
+      integer, intent(in) :: outer_sst, outer_state
+      real(KINDR) :: t(0)
+      integer :: i(0)
+      call set_mc_config(0, t, i, i, i, i, outer_sst, outer_state)
+   end subroutine set_empty_mc_config
+
+   !> dest_ctqmc (CTQMC.F90:901)
+   subroutine dest_ctqmc()
+      integer(c_int) :: st
+      if (c_associated(handle)) then
+         st = w2d_ctqmc_destroy(handle)
+         handle = c_null_ptr
+      end if
+   end subroutine dest_ctqmc
+
+end module MCTQMC
