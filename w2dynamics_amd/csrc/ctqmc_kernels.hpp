@@ -414,17 +414,22 @@ DEV void apply_small(const double *P, const double *Kin, double *Kout, int nt, i
     double dk[NM];
 #pragma unroll
     for (int k = 0; k < NM; ++k) dk[k] = readlane_d(d, k);
+    // columns k >= ns contribute zero: their factor is zeroed and their
+    // addresses clamped, so the unrolled products run without branches
+#pragma unroll
+    for (int k = 0; k < NM; ++k) dk[k] = k < ns ? dk[k] : 0.0;
     double v[PER];
     double m = 0.0;
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
         double acc = 0.0;
-        const int r = ls.rr[u], i = ls.ii[u];
-        if (r < nt) {
+        const int r = ls.rr[u] < nt ? ls.rr[u] : 0, i = ls.ii[u];
 #pragma unroll
-            for (int k = 0; k < NM; ++k)
-                if (k < ns) acc = fma(P[r + k * nt] * dk[k], Kin[k * no + i], acc);
+        for (int k = 0; k < NM; ++k) {
+            const int kk = k < ns ? k : 0;
+            acc = fma(P[r + kk * nt] * dk[k], Kin[kk * no + i], acc);
         }
+        acc = ls.rr[u] < nt ? acc : 0.0;
         v[u] = acc;
         m = fmax(m, fabs(acc));
     }
@@ -524,23 +529,31 @@ DEV bool ket_walk(Ctx &c, double *&K, double *&Kw, int &s, double &lg, int no, d
     double t = t0;
     Work w;
     if constexpr (NM <= 16) {
+        // the next operator's time, code, target block and propagator factors
+        // are read and computed before the current step's barrier
         const LaneSlots<NM> ls(no);
+        double tj = n > 0 ? otau[0] : t1;
+        int cj = n > 0 ? ocode[0] : 0;
+        double d = prop_lane(c, s, tj - t, lg);
         for (int j = 0; j < n; ++j) {
-            const double tj = otau[j];
-            const int cj = ocode[j];
             const int ts = c.conn(s, cj, false);
             if (ts < 0) return w.commit(c), false;
-            const double d = prop_lane(c, s, tj - t, lg);
             const int ns = c.sz[s], nt = c.sz[ts];
+            const double *P = c.opblock(s, cj, false);
+            const double tn = j + 1 < n ? otau[j + 1] : t1;
+            const int cn = j + 1 < n ? ocode[j + 1] : 0;
+            const double dn = prop_lane(c, ts, tn - tj, lg);
             w.op(nt, ns, no);
-            apply_small<NM>(c.opblock(s, cj, false), K, Kw, nt, ns, no, d, ls, lg);
+            apply_small<NM>(P, K, Kw, nt, ns, no, d, ls, lg);
             double *x = K;
             K = Kw;
             Kw = x;
             s = ts;
             t = tj;
+            tj = tn;
+            cj = cn;
+            d = dn;
         }
-        const double d = prop_lane(c, s, t1 - t, lg);
         finish_small<NM>(K, c.sz[s], no, d, ls, lg);
     } else {
         for (int j = 0; j < n; ++j) {
@@ -574,21 +587,27 @@ DEV void bra_walk(Ctx &c, double *&B, double *&Bw, int &s, double &lg, int no, d
     Work w;
     if constexpr (NM <= 16) {
         const LaneSlots<NM> ls(no);
+        double tj = n > 0 ? otau[n - 1] : t0;
+        int cj = n > 0 ? ocode[n - 1] : 0;
+        double d = prop_lane(c, s, t - tj, lg);
         for (int j = n - 1; j >= 0; --j) {
-            const double tj = otau[j];
-            const int cj = ocode[j];
             const int ts = c.conn(s, cj, true);
-            const double d = prop_lane(c, s, t - tj, lg);
             const int ns = c.sz[s], nt = c.sz[ts];
+            const double *P = c.opblock(s, cj, true);
+            const double tn = j > 0 ? otau[j - 1] : t0;
+            const int cn = j > 0 ? ocode[j - 1] : 0;
+            const double dn = prop_lane(c, ts, tj - tn, lg);
             w.op(nt, ns, no);
-            apply_small<NM>(c.opblock(s, cj, true), B, Bw, nt, ns, no, d, ls, lg);
+            apply_small<NM>(P, B, Bw, nt, ns, no, d, ls, lg);
             double *x = B;
             B = Bw;
             Bw = x;
             s = ts;
             t = tj;
+            tj = tn;
+            cj = cn;
+            d = dn;
         }
-        const double d = prop_lane(c, s, t - t0, lg);
         finish_small<NM>(B, c.sz[s], no, d, ls, lg);
     } else {
         for (int j = n - 1; j >= 0; --j) {
@@ -832,12 +851,25 @@ DEV int pairs_from(const Ctx &c, const double *tau, const int *code, int n, int 
     if (te < c.h.tauwin_min || te > c.h.tauwin_max) return 0;
     const int ce = code[j];
     const int fe = c.fl(ce), ae = op_ca(ce);
+    // partners: later operators up to min(window end, te + taudiff_max); the
+    // list is read four entries at a time so that the LDS loads overlap
+    const double lim = fmin(c.h.tauwin_max, te + c.h.taudiff_max);
     int cnt = 0;
-    for (int k = j + 1; k < n; ++k) {
-        const double tk = tau[k];
-        if (tk > c.h.tauwin_max || tk > te + c.h.taudiff_max) break;
-        const int ck = code[k];
-        if (c.fl(ck) == fe && op_ca(ck) != ae) ++cnt;
+    for (int k = j + 1; k < n; k += 4) {
+        double tk[4];
+        int ck[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            tk[u] = k + u < n ? tau[k + u] : DBL_MAX;
+            ck[u] = k + u < n ? code[k + u] : 0;
+        }
+        bool stop = false;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            stop = stop || tk[u] > lim;
+            if (!stop && c.fl(ck[u]) == fe && op_ca(ck[u]) != ae) ++cnt;
+        }
+        if (stop) break;
     }
     return cnt;
 }
