@@ -153,6 +153,8 @@ int layout_and_alloc(w2d_ctqmc *h, int nbound)
     L.l_ev = lds((long)m.nstates * 8);
     L.l_wtau = lds((long)lc * 8);
     L.l_wcode = lds((long)lc * 4);
+    L.l_ttau = lds((long)lc * 8);
+    L.l_tcode = lds((long)lc * 4);
     L.l_dvec = lds((long)nm * 8);
     const long qn = std::max<long>(kc + 2, m.nsst + 1);
     L.l_q = lds(qn * 8);
@@ -162,7 +164,7 @@ int layout_and_alloc(w2d_ctqmc *h, int nbound)
     L.l_pl = lds((long)(2 * nm + 2) * 8);
     L.l_k = nm <= 16 ? lds(2L * nm * nm * 8) : -1;
     const long opsb = (long)m.ops.size() * 8;
-    L.l_ops = (q + opsb <= 60 * 1024) ? lds(opsb) : -1;
+    L.l_ops = (q + opsb <= 64 * 1024) ? lds(opsb) : -1;
     L.lds_bytes = q;
     if (L.lds_bytes > 64 * 1024) return W2D_ERR_CAPACITY;
     if (h->chains) {

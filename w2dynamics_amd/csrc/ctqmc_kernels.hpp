@@ -241,8 +241,10 @@ struct Ctx {
     int *fcs, *fas;      // physical row (creator) / column (annihilator) slot of each rank
     int *bndsst;
     // work buffers
-    double *wtau;        // operator list (LDS), lcap entries
+    double *wtau;        // window list (LDS): operators [wia, wib) while h.wvalid
     int *wcode;
+    double *ttau;        // trial list / staging buffer (LDS), same capacity
+    int *tcode;
     double *K0, *K1;     // trace work matrices (LDS for small blocks)
     double *dvec, *Q, *R, *colb, *rowb, *pl;
     int nm;              // padded block size used for matrix strides
@@ -326,13 +328,36 @@ DEV int upper_bound_tau(const double *tau, int n, double x)
     return lo;
 }
 
-// indices [wia, wib) of the operators inside [tauwin_min, tauwin_max]
-DEV void ensure_window_range(Ctx &c)
+// indices [wia, wib) of the operators inside [tauwin_min, tauwin_max] and
+// their copy in the LDS window list; kept across moves until the window
+// shifts or a global / tau-shift move rewrites the configuration.  Returns
+// false if the window does not fit the LDS list.
+DEV bool ensure_window(Ctx &c)
 {
-    if (c.h.wvalid) return;
-    c.h.wia = lower_bound_tau(c.tau, c.h.nop, c.h.tauwin_min);
-    c.h.wib = upper_bound_tau(c.tau, c.h.nop, c.h.tauwin_max);
+    if (c.h.wvalid) return true;
+    const int ia = lower_bound_tau(c.tau, c.h.nop, c.h.tauwin_min);
+    const int ib = upper_bound_tau(c.tau, c.h.nop, c.h.tauwin_max);
+    if (ib - ia + 2 > c.L.lcap_lds) return false;
+    for (int j = lane_id(); j < ib - ia; j += WAVE) {
+        c.wtau[j] = c.tau[ia + j];
+        c.wcode[j] = c.code[ia + j];
+    }
+    c.h.wia = ia;
+    c.h.wib = ib;
     c.h.wvalid = 1;
+    wsync();
+    return true;
+}
+
+// the trial list becomes the window list (after an accepted add / remove)
+DEV void swap_lists(Ctx &c)
+{
+    double *t = c.wtau;
+    c.wtau = c.ttau;
+    c.ttau = t;
+    int *k = c.wcode;
+    c.wcode = c.tcode;
+    c.tcode = k;
 }
 
 // ---------------------------------------------------------------------------
@@ -635,8 +660,8 @@ DEV int stage_chunk(Ctx &c, int j0, int j1)
 {
     const int n = min(j1 - j0, c.L.lcap_lds);
     for (int j = lane_id(); j < n; j += WAVE) {
-        c.wtau[j] = c.tau[j0 + j];
-        c.wcode[j] = c.code[j0 + j];
+        c.ttau[j] = c.tau[j0 + j];
+        c.tcode[j] = c.code[j0 + j];
     }
     wsync();
     return n;
@@ -650,14 +675,14 @@ DEV bool ket_walk_global(Ctx &c, double *&K, double *&Kw, int &s, double &lg, in
     double t = t0;
     for (int b = j0; b < j1;) {
         int n = stage_chunk(c, b, j1);
-        double tend = (b + n < j1) ? c.wtau[n - 1] : t1;
+        double tend = (b + n < j1) ? c.ttau[n - 1] : t1;
         // walk the chunk; the final propagation of a non-last chunk is zero
-        if (!ket_walk<NM>(c, K, Kw, s, lg, no, t, c.wtau, c.wcode, n, tend)) return false;
+        if (!ket_walk<NM>(c, K, Kw, s, lg, no, t, c.ttau, c.tcode, n, tend)) return false;
         t = tend;
         b += n;
         wsync();
     }
-    if (j0 >= j1) ket_walk<NM>(c, K, Kw, s, lg, no, t0, c.wtau, c.wcode, 0, t1);
+    if (j0 >= j1) ket_walk<NM>(c, K, Kw, s, lg, no, t0, c.ttau, c.tcode, 0, t1);
     return true;
 }
 
@@ -669,13 +694,13 @@ DEV void bra_walk_global(Ctx &c, double *&B, double *&Bw, int &s, double &lg, in
     for (int e = j1; e > j0;) {
         int b = max(j0, e - c.L.lcap_lds);
         int n = stage_chunk(c, b, e);
-        double tend = (b > j0) ? c.wtau[0] : t0;
-        bra_walk<NM>(c, B, Bw, s, lg, no, t, c.wtau, c.wcode, n, tend);
+        double tend = (b > j0) ? c.ttau[0] : t0;
+        bra_walk<NM>(c, B, Bw, s, lg, no, t, c.ttau, c.tcode, n, tend);
         t = tend;
         e = b;
         wsync();
     }
-    if (j0 >= j1) bra_walk<NM>(c, B, Bw, s, lg, no, t1, c.wtau, c.wcode, 0, t0);
+    if (j0 >= j1) bra_walk<NM>(c, B, Bw, s, lg, no, t1, c.ttau, c.tcode, 0, t0);
 }
 
 DEV void set_identity(double *K, int no)
@@ -825,21 +850,10 @@ DEV bool window_trace(Ctx &c, int nl, double &tl, double &ts)
     int s = c.h.kst_sst;
     copy_mat(K, c.kst, c.sz[s] * no);
     double lg = c.h.kst_log;
-    if (!ket_walk<NM>(c, K, Kw, s, lg, no, ta, c.wtau, c.wcode, nl, tb)) return false;
+    if (!ket_walk<NM>(c, K, Kw, s, lg, no, ta, c.ttau, c.tcode, nl, tb)) return false;
     if (s != c.bndsst[top]) return false;
     contract(c, K, c.bnd + (long)top * c.nm * c.nm, s, no, lg + c.bndlog[top], tl, ts);
     return true;
-}
-
-// load the window's operators [wia, wib) into the LDS list
-DEV void load_window(Ctx &c)
-{
-    const int ia = c.h.wia, nw = c.h.wib - c.h.wia;
-    for (int j = lane_id(); j < nw; j += WAVE) {
-        c.wtau[j] = c.tau[ia + j];
-        c.wcode[j] = c.code[ia + j];
-    }
-    wsync();
 }
 
 // ---------------------------------------------------------------------------
@@ -893,10 +907,10 @@ DEV int npairs_list(const Ctx &c, const double *tau, const int *code, int n)
 DEV double config_sign(Ctx &c)
 {
     const int n = c.h.nop, nb = c.M.nb, nf = c.M.nf;
-    // stage the operator codes in LDS (the list buffer is free here)
+    // stage the operator codes in LDS (the trial buffer is free here)
     const int *codes = c.code;
     if (n <= 2 * c.L.lcap_lds) {
-        int *st = reinterpret_cast<int *>(c.wtau);
+        int *st = reinterpret_cast<int *>(c.ttau);
         for (int j = lane_id(); j < n; j += WAVE) st[j] = c.code[j];
         wsync();
         codes = st;
@@ -1155,19 +1169,21 @@ DEV void store_part(Ctx &c, int no)
 }
 
 // write the LDS trial list into the global operator list at wia (its length
-// nl replaces the window's nw operators: the tail [wib, nop) moves first)
+// nl replaces the window's nw operators: the tail [wib, nop) moves first); the
+// trial list then serves as the window list
 DEV void commit_window(Ctx &c, int nl)
 {
     const int nw = c.h.wib - c.h.wia;
     block_move(c.tau, c.h.wib, c.h.nop, nl - nw);
     block_move(c.code, c.h.wib, c.h.nop, nl - nw);
     for (int j = lane_id(); j < nl; j += WAVE) {
-        c.tau[c.h.wia + j] = c.wtau[j];
-        c.code[c.h.wia + j] = c.wcode[j];
+        c.tau[c.h.wia + j] = c.ttau[j];
+        c.code[c.h.wia + j] = c.tcode[j];
     }
     wsync();
     c.h.nop += nl - nw;
     c.h.wib = c.h.wia + nl;
+    swap_lists(c);
 }
 
 // StepAdd (CTQMC.F90:3353-3612), change_outer = false
@@ -1188,14 +1204,13 @@ DEV int step_add(Ctx &c)
     const int f = orb + M.nb * spin;
     if (t2 == tref) return c.h.qn_rej++, OUT_QN;
     const int k = c.h.kf[f];
-    ensure_window_range(c);
-    const int nw = c.h.wib - c.h.wia;
-    if (k + 1 >= c.L.kcap || c.h.nop + 2 > c.L.cap || nw + 2 > c.L.lcap_lds) {
+    if (!ensure_window(c) || c.h.wib - c.h.wia + 2 > c.L.lcap_lds || k + 1 >= c.L.kcap ||
+        c.h.nop + 2 > c.L.cap) {
         c.h.err = ERR_CAP;
         return OUT_REJ;
     }
+    const int nw = c.h.wib - c.h.wia;
     ensure_cache<NM>(c);
-    load_window(c);
     // equal times (process_OperAdd, Trace.F90:4287-4380) and insertion points
     const double ta = tref, tc = t2;
     const double t1 = fmin(ta, tc), t2o = fmax(ta, tc);
@@ -1211,16 +1226,17 @@ DEV int step_add(Ctx &c)
     if (wave_isum(eq)) return c.h.qn_rej++, OUT_QN;
     q1 = wave_isum(q1);
     q2 = wave_isum(q2);
-    // trial list: [q2, nw) up by 2, [q1, q2) up by 1, new operators in the gaps
-    block_move(c.wtau, q2, nw, 2);
-    block_move(c.wcode, q2, nw, 2);
-    block_move(c.wtau, q1, q2, 1);
-    block_move(c.wcode, q1, q2, 1);
-    if (lane_id() == 0) {
-        c.wtau[q1] = t1;
-        c.wcode[q1] = code1;
-        c.wtau[q2 + 1] = t2o;
-        c.wcode[q2 + 1] = code2;
+    // trial list: the window with t1 before index q1 and t2o before index q2
+    for (int j = lane_id(); j < nw + 2; j += WAVE) {
+        double x;
+        int cd;
+        if (j < q1) x = c.wtau[j], cd = c.wcode[j];
+        else if (j == q1) x = t1, cd = code1;
+        else if (j <= q2) x = c.wtau[j - 1], cd = c.wcode[j - 1];
+        else if (j == q2 + 1) x = t2o, cd = code2;
+        else x = c.wtau[j - 2], cd = c.wcode[j - 2];
+        c.ttau[j] = x;
+        c.tcode[j] = cd;
     }
     wsync();
     const int nl = nw + 2;
@@ -1249,7 +1265,7 @@ DEV int step_add(Ctx &c)
     const double rnd = grnd(c.rng);
     // NPairs of the new configuration: pairs with both ends in the window all
     // lie in the trial list
-    const int np = npairs_list(c, c.wtau, c.wcode, nl);
+    const int np = npairs_list(c, c.ttau, c.tcode, nl);
     const double wr = exp((tl - c.h.trace_log) + dr_log);
     const double prob = fabs(((1.0 * tdf) * (double)(2 * M.nb)) / (double)np * wr);
     if (!(rnd < prob)) return OUT_REJ;
@@ -1297,19 +1313,18 @@ DEV int step_rem(Ctx &c)
 {
     const DevModel &M = c.M;
     c.h.try_rem++;
-    ensure_window_range(c);
-    const int nw = c.h.wib - c.h.wia;
-    if (nw > c.L.lcap_lds) {
+    if (!ensure_window(c)) {
         c.h.err = ERR_CAP;
         return OUT_REJ;
     }
+    const int nw = c.h.wib - c.h.wia;
     ensure_cache<NM>(c);
-    load_window(c);
     // pair counts per window operator (the window holds every removable pair)
-    int np = 0;
+    int np = 0, cnt0 = 0;   // cnt0: this lane's count in the first 64 operators
     for (int j0 = 0; j0 < nw; j0 += WAVE) {
         const int j = j0 + lane_id();
         const int cnt = j < nw ? pairs_from(c, c.wtau, c.wcode, nw, j) : 0;
+        if (j0 == 0) cnt0 = cnt;
         np += wave_isum(cnt);
     }
     if (np <= 0) return c.h.qn_rej++, OUT_QN;
@@ -1319,7 +1334,7 @@ DEV int step_rem(Ctx &c)
         int base = 0;
         for (int j0 = 0; j0 < nw && w1 < 0; j0 += WAVE) {
             const int j = j0 + lane_id();
-            const int cnt = j < nw ? pairs_from(c, c.wtau, c.wcode, nw, j) : 0;
+            const int cnt = j0 == 0 ? cnt0 : (j < nw ? pairs_from(c, c.wtau, c.wcode, nw, j) : 0);
             int tot;
             const int ex = wave_excl_scan(cnt, &tot);
             int found = (pick > base + ex && pick <= base + ex + cnt) ? j : -1;
@@ -1352,12 +1367,14 @@ DEV int step_rem(Ctx &c)
     const double hprf = (double)np /
                         ((c.h.tauwin_max - c.h.tauwin_min) *
                          (fmin(ta + c.h.taudiff_max, c.h.tauwin_max) - fmax(ta - c.h.taudiff_max, c.h.tauwin_min)));
-    // trial list without the pair
-    block_move(c.wtau, w1 + 1, w2, -1);
-    block_move(c.wcode, w1 + 1, w2, -1);
-    block_move(c.wtau, w2 + 1, nw, -2);
-    block_move(c.wcode, w2 + 1, nw, -2);
+    // trial list: the window without entries w1 < w2
     const int nl = nw - 2;
+    for (int j = lane_id(); j < nl; j += WAVE) {
+        const int src = j < w1 ? j : (j < w2 - 1 ? j + 1 : j + 2);
+        c.ttau[j] = c.wtau[src];
+        c.tcode[j] = c.wcode[src];
+    }
+    wsync();
     double tl, ts;
     if (!window_trace<NM>(c, nl, tl, ts)) return c.h.qn_rej++, OUT_QN;
     const long fo = (long)f * c.L.kcap;
@@ -1608,6 +1625,7 @@ DEV int step_glob(Ctx &c)
         store_part(c, c.sz[c.h.outer]);
         c.h.cfgsign = -config_sign(c);
         c.h.cache_pos = -1;
+        c.h.wvalid = 0;   // relabelled codes: reload the window list
         c.h.acc_glob++;
         return OUT_ACC;
     }
@@ -2030,6 +2048,8 @@ DEV Ctx make_ctx(const DevModel &M, const ChainLayout &L, char *base, char *lds)
     double *ev = reinterpret_cast<double *>(lds + L.l_ev);
     c.wtau = reinterpret_cast<double *>(lds + L.l_wtau);
     c.wcode = reinterpret_cast<int *>(lds + L.l_wcode);
+    c.ttau = reinterpret_cast<double *>(lds + L.l_ttau);
+    c.tcode = reinterpret_cast<int *>(lds + L.l_tcode);
     c.dvec = reinterpret_cast<double *>(lds + L.l_dvec);
     c.Q = reinterpret_cast<double *>(lds + L.l_q);
     c.R = reinterpret_cast<double *>(lds + L.l_r);
@@ -2069,6 +2089,7 @@ DEV Ctx make_ctx(const DevModel &M, const ChainLayout &L, char *base, char *lds)
     c.sz = sz, c.off = off, c.cn = cn, c.oo = oo, c.ev = ev;
     c.rng.mt = mt;
     c.rng.idx = &hd.mti;
+    hd.wvalid = 0;   // the LDS window list does not survive a launch
     wsync();
     return c;
 }

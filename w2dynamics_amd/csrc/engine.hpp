@@ -59,9 +59,9 @@ struct ChainLayout {
     int cap, kcap, nbound, lcap;
     long kmat;            // max(kcap*kcap, nmax*nmax)
     // dynamic LDS carve-up (byte offsets; l_ops / l_k = -1: kept in global memory)
-    long l_hdr, l_mt, l_sz, l_off, l_cn, l_oo, l_ev, l_ops, l_wtau, l_wcode;
+    long l_hdr, l_mt, l_sz, l_off, l_cn, l_oo, l_ev, l_ops, l_wtau, l_wcode, l_ttau, l_tcode;
     long l_dvec, l_q, l_r, l_col, l_row, l_pl, l_k, lds_bytes;
-    int lcap_lds;         // operators the LDS list holds (window + 2)
+    int lcap_lds;         // operators each LDS list holds (window list, trial list)
 };
 
 }  // namespace w2d
