@@ -155,6 +155,33 @@ def cpu_baseline(prob, tdm, seed, nwarm, nmeas, ncorr):
                 wall_s=wall)
 
 
+def copy_probe(mib: int):
+    """Plain device-to-device copies of known size: the FETCH_SIZE / WRITE_SIZE
+    counters of these dispatches calibrate the counters of the sweep kernel."""
+    import torch
+    n = mib * (1 << 20) // 8
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8,
+                      "dispatches": 5}))
+
+
+def traffic_from_profiles(workload: str):
+    """HBM bytes per sweep-kernel dispatch measured with rocprofv3 --pmc for this
+    workload (profiles/sweep_kernel_traffic_<workload>.txt), if recorded."""
+    path = os.path.join(ROOT, "profiles", "sweep_kernel_traffic_%s.txt" % workload)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_dispatch:"):
+                return float(line.split(":", 1)[1])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -166,7 +193,12 @@ def main():
     ap.add_argument("--seed", type=int, default=43890)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run 5 device copies of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe_mib > 0:
+        copy_probe(args.copy_probe_mib)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the CT-HYB chains of one handle live on one device "
               "(multi-GPU sharding is a design note, not built); run with --gpus 1",
@@ -217,7 +249,7 @@ def main():
         "batched_vs_plain_inv_err2_ratio": (inv_err2_s / plain["gtau_inv_err2_per_s"]) if plain else None,
         "batched_vs_plain_moves_ratio": (value / plain["moves_per_s"]) if plain else None,
         "roofline": {"bound": "hbm", "achieved": ach_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": ach_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "frac": ach_gbs / HBM_PEAK_GBS, "traffic": traffic_from_profiles(args.config),
                      "kernel": "sweep_kernel", "launches": b["launches"],
                      "avg_launch_ms": b["kernel_ms"] / max(1, b["launches"]),
                      "alg_bytes_per_launch": b["alg_bytes"] / max(1, b["launches"]),
