@@ -210,15 +210,16 @@ int read_headers(w2d_ctqmc *h, std::vector<ChainHdr> &hd)
 {
     hd.resize(h->nchains);
     HIPCHK(hipStreamSynchronize(h->stream));
-    for (int c = 0; c < h->nchains; ++c)
-        HIPCHK(hipMemcpy(&hd[c], h->chains + (long)c * h->L.stride, sizeof(ChainHdr), hipMemcpyDeviceToHost));
+    // one strided copy: the header leads every chain block
+    HIPCHK(hipMemcpy2D(hd.data(), sizeof(ChainHdr), h->chains, (size_t)h->L.stride, sizeof(ChainHdr),
+                       (size_t)h->nchains, hipMemcpyDeviceToHost));
     return W2D_OK;
 }
 
 int write_headers(w2d_ctqmc *h, const std::vector<ChainHdr> &hd)
 {
-    for (int c = 0; c < h->nchains; ++c)
-        HIPCHK(hipMemcpy(h->chains + (long)c * h->L.stride, &hd[c], sizeof(ChainHdr), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy2D(h->chains, (size_t)h->L.stride, hd.data(), sizeof(ChainHdr), sizeof(ChainHdr),
+                       (size_t)h->nchains, hipMemcpyHostToDevice));
     return W2D_OK;
 }
 
@@ -233,20 +234,22 @@ int first_error(const std::vector<ChainHdr> &hd)
 // the TaudiffMax parameter, then the initial outer-superstate draw on the GPU
 int init_chains(w2d_ctqmc *h)
 {
-    std::vector<uint32_t> mt(624);
+    std::vector<ChainHdr> hds(h->nchains);
+    std::vector<uint32_t> mt(624L * h->nchains);
     for (int c = 0; c < h->nchains; ++c) {
-        ChainHdr hd;
+        ChainHdr &hd = hds[c];
         std::memset(&hd, 0, sizeof(hd));
         hd.det_sign = 1.0;
         hd.trace_sign = 1.0;
         hd.cfgsign = -1.0;
         hd.mti = 624;
         init_window(hd, h->taudiff_param, h->beta);
-        char *base = h->chains + (long)c * h->L.stride;
-        HIPCHK(hipMemcpy(base, &hd, sizeof(hd), hipMemcpyHostToDevice));
-        seed_mt(mt.data(), h->seed0 + (uint32_t)c);
-        HIPCHK(hipMemcpy(base + h->L.mt, mt.data(), 624 * 4, hipMemcpyHostToDevice));
+        seed_mt(mt.data() + 624L * c, h->seed0 + (uint32_t)c);
     }
+    int st0 = write_headers(h, hds);
+    if (st0) return st0;
+    HIPCHK(hipMemcpy2D(h->chains + h->L.mt, (size_t)h->L.stride, mt.data(), 624 * 4, 624 * 4,
+                       (size_t)h->nchains, hipMemcpyHostToDevice));
     DISPATCH(launch_init, h);
     HIPCHK(hipGetLastError());
     std::vector<ChainHdr> hd;
@@ -456,8 +459,8 @@ int w2d_ctqmc_measure(w2d_ctqmc *h, int64_t nmeas)
     if ((st = write_headers(h, hd))) return st;
     // clear the accumulators of every chain
     const w2d::ChainLayout &L = h->L;
-    for (int c = 0; c < h->nchains; ++c)
-        HIPCHK(hipMemsetAsync(h->chains + (long)c * L.stride + L.gtau, 0, L.stride - L.gtau, h->stream));
+    HIPCHK(hipMemset2DAsync(h->chains + L.gtau, (size_t)L.stride, 0, (size_t)(L.stride - L.gtau),
+                            (size_t)h->nchains, h->stream));
     st = run_moves(h, nmeas * h->ncorr, h->ncorr);
     h->measured = (st == W2D_OK);
     return st;
@@ -490,21 +493,25 @@ int w2d_ctqmc_get_accumulators(w2d_ctqmc *h, double *gtau, double *gleg, double 
     std::vector<ChainHdr> hd;
     int st = read_headers(h, hd);
     if (st) return st;
+    // strided gathers: one copy per array over all chains
+    auto gather = [&](double *dst, long off, size_t bytes) -> int {
+        HIPCHK(hipMemcpy2D(dst, bytes, h->chains + off, (size_t)L.stride, bytes, (size_t)h->nchains,
+                           hipMemcpyDeviceToHost));
+        return W2D_OK;
+    };
+    if (gtau && (st = gather(gtau, L.gtau, (size_t)nf * h->ntau * 8))) return st;
+    if (gleg && (st = gather(gleg, L.gleg, (size_t)nf * h->nleg * 8))) return st;
+    if (histo && (st = gather(histo, L.histo, (size_t)nf * (h->maxhisto + 1) * 8))) return st;
+    if (outer && (st = gather(outer, L.outer, (size_t)m.nsst * 8))) return st;
+    std::vector<double> dmall;
+    if (dm) {
+        dmall.resize((size_t)h->dm.dmsize * h->nchains);
+        if ((st = gather(dmall.data(), L.dm, (size_t)h->dm.dmsize * 8))) return st;
+    }
     for (int c = 0; c < h->nchains; ++c) {
-        char *base = h->chains + (long)c * L.stride;
-        if (gtau)
-            HIPCHK(hipMemcpy(gtau + (long)c * nf * h->ntau, base + L.gtau, (size_t)nf * h->ntau * 8,
-                             hipMemcpyDeviceToHost));
-        if (gleg)
-            HIPCHK(hipMemcpy(gleg + (long)c * nf * h->nleg, base + L.gleg, (size_t)nf * h->nleg * 8,
-                             hipMemcpyDeviceToHost));
-        if (histo)
-            HIPCHK(hipMemcpy(histo + (long)c * nf * (h->maxhisto + 1), base + L.histo,
-                             (size_t)nf * (h->maxhisto + 1) * 8, hipMemcpyDeviceToHost));
-        if (outer)
-            HIPCHK(hipMemcpy(outer + (long)c * m.nsst, base + L.outer, (size_t)m.nsst * 8, hipMemcpyDeviceToHost));
         if (dm) {
-            HIPCHK(hipMemcpy(blk.data(), base + L.dm, blk.size() * 8, hipMemcpyDeviceToHost));
+            std::copy(dmall.begin() + (long)c * h->dm.dmsize, dmall.begin() + (long)(c + 1) * h->dm.dmsize,
+                      blk.begin());
             double *D = dm + (long)c * m.nstates * m.nstates;
             std::fill(D, D + (long)m.nstates * m.nstates, 0.0);
             long o = 0;
