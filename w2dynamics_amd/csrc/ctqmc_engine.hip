@@ -105,7 +105,7 @@ int layout_and_alloc(w2d_ctqmc *h, int nbound)
     L.lcap = L.cap + 2;
     L.nbound = nbound;
     L.kmat = std::max<long>((long)kc * kc, (long)nm * nm);
-    long o = align256(sizeof(ChainHdr));
+    long o = 0;
     auto take = [&](long bytes) {
         long at = o;
         o = align256(o + bytes);
@@ -163,16 +163,20 @@ int layout_and_alloc(w2d_ctqmc *h, int nbound)
     L.l_row = lds((long)(kc + 2) * 8);
     L.l_pl = lds((long)(2 * nm + 2) * 8);
     L.l_k = nm <= 16 ? lds(2L * nm * nm * 8) : -1;
+    L.l_legc = lds(2L * std::max(h->nleg, 1) * 8);
     const long opsb = (long)m.ops.size() * 8;
-    L.l_ops = (q + opsb <= 64 * 1024) ? lds(opsb) : -1;
+    // blocks of up to 4 states: the kernels read operator blocks from LDS only
+    L.l_ops = (nm <= 4 || q + opsb <= 64 * 1024) ? lds(opsb) : -1;
     L.lds_bytes = q;
     if (L.lds_bytes > 64 * 1024) return W2D_ERR_CAPACITY;
     if (h->chains) {
         HIPCHK(hipFree(h->chains));
         h->chains = nullptr;
     }
-    HIPCHK(hipMalloc(&h->chains, (size_t)L.stride * h->nchains));
-    HIPCHK(hipMemset(h->chains, 0, (size_t)L.stride * h->nchains));
+    L.hdrs = L.stride * h->nchains;
+    const size_t total = (size_t)L.hdrs + sizeof(ChainHdr) * (size_t)h->nchains;
+    HIPCHK(hipMalloc(&h->chains, total));
+    HIPCHK(hipMemset(h->chains, 0, total));
     h->nwin_alloc = nbound;
     return W2D_OK;
 }
@@ -206,20 +210,22 @@ void launch_setconfig(w2d_ctqmc *h, int chain)
     default: fn<128>(__VA_ARGS__); break;    \
     }
 
+// device address of chain c's header
+char *hdr_of(w2d_ctqmc *h, int c) { return h->chains + h->L.hdrs + (long)c * (long)sizeof(ChainHdr); }
+
 int read_headers(w2d_ctqmc *h, std::vector<ChainHdr> &hd)
 {
     hd.resize(h->nchains);
     HIPCHK(hipStreamSynchronize(h->stream));
-    // one strided copy: the header leads every chain block
-    HIPCHK(hipMemcpy2D(hd.data(), sizeof(ChainHdr), h->chains, (size_t)h->L.stride, sizeof(ChainHdr),
-                       (size_t)h->nchains, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(hd.data(), h->chains + h->L.hdrs, sizeof(ChainHdr) * (size_t)h->nchains,
+                     hipMemcpyDeviceToHost));
     return W2D_OK;
 }
 
 int write_headers(w2d_ctqmc *h, const std::vector<ChainHdr> &hd)
 {
-    HIPCHK(hipMemcpy2D(h->chains, (size_t)h->L.stride, hd.data(), sizeof(ChainHdr), sizeof(ChainHdr),
-                       (size_t)h->nchains, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->chains + h->L.hdrs, hd.data(), sizeof(ChainHdr) * (size_t)h->nchains,
+                     hipMemcpyHostToDevice));
     return W2D_OK;
 }
 
@@ -581,7 +587,7 @@ int w2d_ctqmc_get_config(w2d_ctqmc *h, int chain, int cap, int *n, double *tau, 
     HIPCHK(hipStreamSynchronize(h->stream));
     char *base = h->chains + (long)chain * h->L.stride;
     ChainHdr x;
-    HIPCHK(hipMemcpy(&x, base, sizeof(x), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&x, hdr_of(h, chain), sizeof(x), hipMemcpyDeviceToHost));
     *n = x.nop;
     if (outer_sst) *outer_sst = x.outer;
     if (weights) {
@@ -624,11 +630,11 @@ int w2d_ctqmc_set_config(w2d_ctqmc *h, int chain, int n, const double *tau, cons
     char *base = h->chains + (long)chain * h->L.stride;
     HIPCHK(hipStreamSynchronize(h->stream));
     ChainHdr x;
-    HIPCHK(hipMemcpy(&x, base, sizeof(x), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&x, hdr_of(h, chain), sizeof(x), hipMemcpyDeviceToHost));
     x.nop = n;
     x.outer = outer_sst;
     x.err = 0;
-    HIPCHK(hipMemcpy(base, &x, sizeof(x), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(hdr_of(h, chain), &x, sizeof(x), hipMemcpyHostToDevice));
     if (n > 0) {
         HIPCHK(hipMemcpy(base + h->L.tau, tau, (size_t)n * 8, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(base + h->L.code, code.data(), (size_t)n * 4, hipMemcpyHostToDevice));
@@ -636,7 +642,7 @@ int w2d_ctqmc_set_config(w2d_ctqmc *h, int chain, int n, const double *tau, cons
     DISPATCH(launch_setconfig, h, chain);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(h->stream));
-    HIPCHK(hipMemcpy(&x, base, sizeof(x), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&x, hdr_of(h, chain), sizeof(x), hipMemcpyDeviceToHost));
     if (weights) {
         weights[0] = x.trace_log, weights[1] = x.trace_sign;
         weights[2] = x.det_log, weights[3] = x.det_sign, weights[4] = x.cfgsign;
@@ -650,7 +656,7 @@ int w2d_ctqmc_get_minv(w2d_ctqmc *h, int chain, int f, int *k, double *out)
     HIPCHK(hipStreamSynchronize(h->stream));
     char *base = h->chains + (long)chain * h->L.stride;
     ChainHdr x;
-    HIPCHK(hipMemcpy(&x, base, sizeof(x), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&x, hdr_of(h, chain), sizeof(x), hipMemcpyDeviceToHost));
     *k = x.kf[f];
     if (!out) return W2D_OK;
     // gather the physical block into logical (rank) order

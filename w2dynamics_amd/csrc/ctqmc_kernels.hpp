@@ -173,15 +173,8 @@ DEV void mt_regen(uint32_t *mt)
     wsync();
 }
 
-DEV uint32_t mt_next(Rng &r)
+DEV uint32_t mt_temper(uint32_t y)
 {
-    int i = *r.idx;
-    if (i >= 624) {
-        mt_regen(r.mt);
-        i = 0;
-    }
-    uint32_t y = r.mt[i];
-    *r.idx = i + 1;   // all lanes store the same value
     y ^= y >> 11;
     y ^= (y << 7) & 0x9d2c5680u;
     y ^= (y << 15) & 0xefc60000u;
@@ -189,11 +182,35 @@ DEV uint32_t mt_next(Rng &r)
     return y;
 }
 
+DEV uint32_t mt_next(Rng &r)
+{
+    int i = *r.idx;
+    if (i >= 624) {
+        mt_regen(r.mt);
+        i = 0;
+    }
+    const uint32_t y = r.mt[i];
+    *r.idx = i + 1;   // all lanes store the same value
+    return mt_temper(y);
+}
+
+// generate_canonical<double,53>: (g1 + g2 2^32) / 2^64, clamped below 1.  The
+// two words are read together when both are left in the current block.
 DEV double grnd(Rng &r)
 {
-    double sum = (double)mt_next(r);
+    const int i = *r.idx;
+    uint32_t g1, g2;
+    if (i + 2 <= 624) {
+        g1 = mt_temper(r.mt[i]);
+        g2 = mt_temper(r.mt[i + 1]);
+        *r.idx = i + 2;
+    } else {
+        g1 = mt_next(r);
+        g2 = mt_next(r);
+    }
+    double sum = (double)g1;
     const double tmp = 4294967296.0;
-    sum = __dadd_rn(sum, __dmul_rn((double)mt_next(r), tmp));
+    sum = __dadd_rn(sum, __dmul_rn((double)g2, tmp));
     double ret = sum / (tmp * tmp);
     if (ret >= 1.0) ret = 0x1.fffffffffffffp-1;
     return ret;
@@ -229,6 +246,7 @@ struct Ctx {
     const DevModel &M;
     const ChainLayout &L;
     char *base;          // chain block in global memory
+    ChainHdr *ghdr;      // the chain's header in the contiguous header array (HBM)
     ChainHdr &h;         // LDS copy
     Rng rng;
     // model tables (LDS)
@@ -247,6 +265,7 @@ struct Ctx {
     int *tcode;
     double *K0, *K1;     // trace work matrices (LDS for small blocks)
     double *dvec, *Q, *R, *colb, *rowb, *pl;
+    const double *legc;  // Legendre recursion coefficients (LDS)
     int nm;              // padded block size used for matrix strides
     DEV Ctx(const DevModel &m, const ChainLayout &l, char *b, ChainHdr &hd) : M(m), L(l), base(b), h(hd) {}
     DEV double *mi(int f) { return minv + (long)f * L.kcap * L.kcap; }
@@ -404,7 +423,7 @@ DEV double prop_lane(const Ctx &c, int s, double dt, double &lg)
     lg += -(e0 - c.M.egs) * dt;
     const int k = lane_id();
     const double ek = k < ns ? E[k] : e0;
-    return ns > 1 ? exp(-(ek - e0) * dt) : 1.0;
+    return exp(-(ek - e0) * dt);   // exactly 1 beyond the block and for 1-state blocks
 }
 
 // Thresholds outside which a state matrix is renormalised during a walk; in
@@ -436,6 +455,22 @@ DEV void apply_small(const double *P, const double *Kin, double *Kout, int nt, i
                      const LaneSlots<NM> &ls, double &lg)
 {
     constexpr int PER = LaneSlots<NM>::PER;
+    // block and state entries first (independent of the propagator factors),
+    // for blocks up to 4 states held in registers
+    constexpr bool PRE = NM <= 4;
+    double pv[PRE ? PER : 1][PRE ? NM : 1], kv[PRE ? PER : 1][PRE ? NM : 1];
+    if constexpr (PRE) {
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int r = ls.rr[u] < nt ? ls.rr[u] : 0, i = ls.ii[u];
+#pragma unroll
+            for (int k = 0; k < NM; ++k) {
+                const int kk = k < ns ? k : 0;
+                pv[u][k] = P[r + kk * nt];
+                kv[u][k] = Kin[kk * no + i];
+            }
+        }
+    }
     double dk[NM];
 #pragma unroll
     for (int k = 0; k < NM; ++k) dk[k] = readlane_d(d, k);
@@ -452,7 +487,10 @@ DEV void apply_small(const double *P, const double *Kin, double *Kout, int nt, i
 #pragma unroll
         for (int k = 0; k < NM; ++k) {
             const int kk = k < ns ? k : 0;
-            acc = fma(P[r + kk * nt] * dk[k], Kin[kk * no + i], acc);
+            if constexpr (PRE)
+                acc = fma(pv[u][k] * dk[k], kv[u][k], acc);
+            else
+                acc = fma(P[r + kk * nt] * dk[k], Kin[kk * no + i], acc);
         }
         acc = ls.rr[u] < nt ? acc : 0.0;
         v[u] = acc;
@@ -1583,27 +1621,13 @@ DEV int step_glob(Ctx &c)
             dnl += l;
             dns *= s;
         }
-        const int ko = c.h.kf[f];
-        const int *csl = c.fcs + f * kc, *asl = c.fas + f * kc;
-        if (ko == 1) {
-            const double v = c.mi(f)[csl[0] + (long)asl[0] * ld];
-            dol += log(fabs(v));
-            dos *= copysign(1.0, v);
-        } else if (ko > 1) {
-            for (int idx = lane_id(); idx < ko * ko; idx += WAVE) {
-                const int i = idx % ko, j = idx / ko;
-                c.scratch[i + (long)j * ld] = c.mi(f)[csl[i] + (long)asl[j] * ld];
-            }
-            wsync();
-            double l, s;
-            lu_inplace(c.scratch, ko, ld, l, s, piv);
-            dol += l;
-            dos *= s;
-        }
-        const double k3 = (double)k * k * k, ko3 = (double)ko * ko * ko;
-        c.h.alg_flops += (2.0 / 3.0) * (k3 + ko3) * 2.0;
-        c.h.alg_bytes += 8.0 * ((double)k * k + (double)ko * ko);
+        const double k3 = (double)k * k * k;
+        c.h.alg_flops += (2.0 / 3.0) * k3 * 2.0;
+        c.h.alg_bytes += 8.0 * (double)k * k;
     }
+    // det(old M) = 1 / det(old Delta), which the chain carries
+    dol = -c.h.det_log;
+    dos = c.h.det_sign;
     double tl, ts;
     full_trace<NM>(c, c.h.outer, tl, ts);
     const double wr = exp((tl - c.h.trace_log) + (dnl + dol));
@@ -1919,7 +1943,7 @@ __device__ __attribute__((noinline)) void meas_gtau(Ctx &c, double *gtau, double
                 double pl = 0.0, plm1 = 0.0;
                 const int lend = min(M.nleg, l0 + LPASS);
                 for (int l = 0; l < l0; ++l) {
-                    const double plp1 = l == 0 ? 1.0 : (l == 1 ? x : M.legc[2 * (l - 1)] * x * pl - M.legc[2 * (l - 1) + 1] * plm1);
+                    const double plp1 = l == 0 ? 1.0 : (l == 1 ? x : c.legc[2 * (l - 1)] * x * pl - c.legc[2 * (l - 1) + 1] * plm1);
                     plm1 = pl;
                     pl = plp1;
                 }
@@ -1933,7 +1957,7 @@ __device__ __attribute__((noinline)) void meas_gtau(Ctx &c, double *gtau, double
                         const int l = lb + j;
                         double plp1 = 0.0;
                         if (l < lend) {
-                            plp1 = l == 0 ? 1.0 : (l == 1 ? x : M.legc[2 * (l - 1)] * x * pl - M.legc[2 * (l - 1) + 1] * plm1);
+                            plp1 = l == 0 ? 1.0 : (l == 1 ? x : c.legc[2 * (l - 1)] * x * pl - c.legc[2 * (l - 1) + 1] * plm1);
                             plm1 = pl;
                             pl = plp1;
                         }
@@ -2022,7 +2046,7 @@ DEV void meas_misc(Ctx &c, double *histo, double *outer_hist)
 // Map the chain block and carve the dynamic LDS; copies header, RNG state and
 // the model tables into LDS.
 template <int NM>
-DEV Ctx make_ctx(const DevModel &M, const ChainLayout &L, char *base, char *lds)
+DEV Ctx make_ctx(const DevModel &M, const ChainLayout &L, char *base, int chain, char *lds)
 {
     ChainHdr &hd = *reinterpret_cast<ChainHdr *>(lds + L.l_hdr);
     Ctx c(M, L, base, hd);
@@ -2056,15 +2080,22 @@ DEV Ctx make_ctx(const DevModel &M, const ChainLayout &L, char *base, char *lds)
     c.colb = reinterpret_cast<double *>(lds + L.l_col);
     c.rowb = reinterpret_cast<double *>(lds + L.l_row);
     c.pl = reinterpret_cast<double *>(lds + L.l_pl);
-    if (L.l_k >= 0) {
+    {
+        double *lc = reinterpret_cast<double *>(lds + L.l_legc);
+        for (int i = lane_id(); i < 2 * M.nleg; i += WAVE) lc[i] = M.legc[i];
+        c.legc = lc;
+    }
+    // work matrices: LDS for blocks up to 16 states (decided at compile time so
+    // that their accesses are LDS instructions, not generic ones), HBM above
+    if constexpr (NM <= 16) {
         c.K0 = reinterpret_cast<double *>(lds + L.l_k);
-        c.K1 = c.K0 + (long)NM * NM;
     } else {
         c.K0 = reinterpret_cast<double *>(base + L.kwork);
-        c.K1 = c.K0 + (long)NM * NM;
     }
+    c.K1 = c.K0 + (long)NM * NM;
     // header, RNG state and model tables into LDS
-    const int *gh = reinterpret_cast<const int *>(base);
+    c.ghdr = reinterpret_cast<ChainHdr *>(base - (long)chain * L.stride + L.hdrs) + chain;
+    const int *gh = reinterpret_cast<const int *>(c.ghdr);
     int *lh = reinterpret_cast<int *>(&hd);
     for (int i = lane_id(); i < (int)(sizeof(ChainHdr) / 4); i += WAVE) lh[i] = gh[i];
     const uint32_t *gmt = reinterpret_cast<const uint32_t *>(base + L.mt);
@@ -2079,7 +2110,13 @@ DEV Ctx make_ctx(const DevModel &M, const ChainLayout &L, char *base, char *lds)
         oo[i] = (int)M.opoff[i];
     }
     for (int i = lane_id(); i < M.nstates; i += WAVE) ev[i] = M.evals[i];
-    if (L.l_ops >= 0) {
+    // operator blocks: always LDS for blocks up to 4 states (known at compile
+    // time, so the block reads are LDS instructions); LDS or HBM above
+    if constexpr (NM <= 4) {
+        double *lops = reinterpret_cast<double *>(lds + L.l_ops);
+        for (long i = lane_id(); i < M.nops; i += WAVE) lops[i] = M.ops[i];
+        c.opsp = lops;
+    } else if (L.l_ops >= 0) {
         double *lops = reinterpret_cast<double *>(lds + L.l_ops);
         for (long i = lane_id(); i < M.nops; i += WAVE) lops[i] = M.ops[i];
         c.opsp = lops;
@@ -2098,7 +2135,7 @@ DEV void save_ctx(Ctx &c)
 {
     uint32_t *gmt = reinterpret_cast<uint32_t *>(c.base + c.L.mt);
     for (int i = lane_id(); i < 624; i += WAVE) gmt[i] = c.rng.mt[i];
-    int *gh = reinterpret_cast<int *>(c.base);
+    int *gh = reinterpret_cast<int *>(c.ghdr);
     const int *lh = reinterpret_cast<const int *>(&c.h);
     wsync();
     for (int i = lane_id(); i < (int)(sizeof(ChainHdr) / 4); i += WAVE) gh[i] = lh[i];
@@ -2116,7 +2153,7 @@ __global__ void __launch_bounds__(WAVE) sweep_kernel(DevModel M, ChainLayout L, 
     if (only >= 0 && chain != only) return;
     extern __shared__ __align__(16) char lds[];
     char *base = chains + (long)chain * L.stride;
-    Ctx c = make_ctx<NM>(M, L, base, lds);
+    Ctx c = make_ctx<NM>(M, L, base, chain, lds);
     double *gtau = reinterpret_cast<double *>(base + L.gtau);
     double *gleg = reinterpret_cast<double *>(base + L.gleg);
     double *dm = reinterpret_cast<double *>(base + L.dm);
@@ -2155,7 +2192,7 @@ __global__ void __launch_bounds__(WAVE) init_kernel(DevModel M, ChainLayout L, c
 {
     extern __shared__ __align__(16) char lds[];
     char *base = chains + (long)blockIdx.x * L.stride;
-    Ctx c = make_ctx<NM>(M, L, base, lds);
+    Ctx c = make_ctx<NM>(M, L, base, (int)blockIdx.x, lds);
     choose_outer<NM>(c);
     if (!(c.h.trace_log > -DBL_MAX)) c.h.err = 5;
     save_ctx(c);
@@ -2170,7 +2207,7 @@ __global__ void __launch_bounds__(WAVE) setconfig_kernel(DevModel M, ChainLayout
     if ((int)blockIdx.x != only) return;
     extern __shared__ __align__(16) char lds[];
     char *base = chains + (long)blockIdx.x * L.stride;
-    Ctx c = make_ctx<NM>(M, L, base, lds);
+    Ctx c = make_ctx<NM>(M, L, base, (int)blockIdx.x, lds);
     c.h.wpos = 0;
     set_window(c);
     double tl, ts;

@@ -24,7 +24,7 @@ struct DevModel {
     int ntau, nleg, maxhisto, has_szt, meas_dm;
 };
 
-// per-chain scalar state (first member of a chain's memory block)
+// per-chain scalar state (kept contiguously for all chains, after the chain blocks)
 struct ChainHdr {
     int nop, outer, wpos, nwin;
     int err, cache_pos, kst_sst, mti;
@@ -43,6 +43,7 @@ struct ChainHdr {
 // byte offsets of the per-chain arrays inside one chain block
 struct ChainLayout {
     long stride;          // bytes per chain
+    long hdrs;            // offset of the header array (ChainHdr[nchains]) after the chain blocks
     long mt;              // uint32[624]
     long tau, code;       // double[cap], int[cap]: configuration sorted by tau
     long fc, fa;          // double[nf][kcap]: creator / annihilator taus per flavour
@@ -60,7 +61,7 @@ struct ChainLayout {
     long kmat;            // max(kcap*kcap, nmax*nmax)
     // dynamic LDS carve-up (byte offsets; l_ops / l_k = -1: kept in global memory)
     long l_hdr, l_mt, l_sz, l_off, l_cn, l_oo, l_ev, l_ops, l_wtau, l_wcode, l_ttau, l_tcode;
-    long l_dvec, l_q, l_r, l_col, l_row, l_pl, l_k, lds_bytes;
+    long l_dvec, l_q, l_r, l_col, l_row, l_pl, l_k, l_legc, lds_bytes;
     int lcap_lds;         // operators each LDS list holds (window list, trial list)
 };
 
