@@ -53,6 +53,10 @@ def gtau_inv_err2(res):
     n = g.shape[0]
     m = g.mean(0)
     se2 = ((g - m) ** 2).sum(0) / (n * (n - 1))
+    if not float(se2.mean()) > 0.0:
+        print("G(tau) samples carry no spread (sum |G| = %g); 1/err^2 undefined" % float(np.abs(g).sum()),
+              file=sys.stderr)
+        return float("nan")
     return 1.0 / float(se2.mean())
 
 
