@@ -21,6 +21,7 @@ struct w2d_ctqmc {
     w2d::DevModel dm{};
     w2d::ChainLayout L{};
     int nchains = 0, nm = 0, device = 0, kcap = 0, ntau = 0, nleg = 0, maxhisto = 0;
+    int lds_max = 64 * 1024;   // LDS bytes a workgroup may allocate on this device
     int nwin_alloc = 0;
     uint32_t seed0 = 0;
     double beta = 0.0, taudiff_param = -1.0;
@@ -150,6 +151,7 @@ int layout_and_alloc(w2d_ctqmc *h, int nbound)
     L.l_off = lds((long)m.nsst * 4);
     L.l_cn = lds((long)m.nsst * m.nbands * 4 * 4);
     L.l_oo = lds((long)m.nsst * m.nbands * 4 * 4);
+    L.l_tr = lds((long)m.nsst * m.nbands * 4 * 16);
     L.l_ev = lds((long)m.nstates * 8);
     L.l_wtau = lds((long)lc * 8);
     L.l_wcode = lds((long)lc * 4);
@@ -166,9 +168,11 @@ int layout_and_alloc(w2d_ctqmc *h, int nbound)
     L.l_legc = lds(2L * std::max(h->nleg, 1) * 8);
     const long opsb = (long)m.ops.size() * 8;
     // blocks of up to 4 states: the kernels read operator blocks from LDS only
+    // operator blocks join LDS when the total stays within 64 KB (more would
+    // cut the number of chains resident per CU); blocks of up to 4 states always
     L.l_ops = (nm <= 4 || q + opsb <= 64 * 1024) ? lds(opsb) : -1;
     L.lds_bytes = q;
-    if (L.lds_bytes > 64 * 1024) return W2D_ERR_CAPACITY;
+    if (L.lds_bytes > h->lds_max) return W2D_ERR_CAPACITY;
     if (h->chains) {
         HIPCHK(hipFree(h->chains));
         h->chains = nullptr;
@@ -181,9 +185,19 @@ int layout_and_alloc(w2d_ctqmc *h, int nbound)
     return W2D_OK;
 }
 
+// allow the dynamic LDS of this handle (above 64 KB on devices that have it)
+template <typename K>
+void allow_lds(K kernel, const w2d_ctqmc *h)
+{
+    if (h->L.lds_bytes > 64 * 1024)
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->L.lds_bytes);
+}
+
 template <int NM>
 void launch_init(w2d_ctqmc *h)
 {
+    allow_lds(w2d::init_kernel<NM>, h);
     hipLaunchKernelGGL(w2d::init_kernel<NM>, dim3(h->nchains), dim3(w2d::WAVE), (unsigned)h->L.lds_bytes, h->stream, h->dm, h->L,
                        h->chains);
 }
@@ -191,6 +205,7 @@ void launch_init(w2d_ctqmc *h)
 template <int NM>
 void launch_sweep(w2d_ctqmc *h, long long nmoves, long long ncorr, int only, w2d_move_record *rec)
 {
+    allow_lds(w2d::sweep_kernel<NM>, h);
     hipLaunchKernelGGL(w2d::sweep_kernel<NM>, dim3(h->nchains), dim3(w2d::WAVE), (unsigned)h->L.lds_bytes, h->stream, h->dm, h->L,
                        h->chains, nmoves, ncorr, only, rec);
 }
@@ -198,6 +213,7 @@ void launch_sweep(w2d_ctqmc *h, long long nmoves, long long ncorr, int only, w2d
 template <int NM>
 void launch_setconfig(w2d_ctqmc *h, int chain)
 {
+    allow_lds(w2d::setconfig_kernel<NM>, h);
     hipLaunchKernelGGL(w2d::setconfig_kernel<NM>, dim3(h->nchains), dim3(w2d::WAVE), (unsigned)h->L.lds_bytes, h->stream, h->dm,
                        h->L, h->chains, chain);
 }
@@ -353,6 +369,7 @@ int w2d_ctqmc_create(const char *params, int nbands, int nftau, const double *u_
     else if (m.nmax <= 128) h->nm = 128;
     else return W2D_ERR_ARG;
     HIPCHK(hipSetDevice(device));
+    HIPCHK(hipDeviceGetAttribute(&h->lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
     HIPCHK(hipStreamCreate(&h->stream));
     HIPCHK(hipEventCreate(&h->ev0));
     HIPCHK(hipEventCreate(&h->ev1));

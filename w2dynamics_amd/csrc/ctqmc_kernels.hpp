@@ -251,6 +251,7 @@ struct Ctx {
     Rng rng;
     // model tables (LDS)
     const int *sz, *off, *cn, *oo;
+    const int4 *tr;      // per (superstate, operator): target, target size, block offset, target energies offset
     const double *ev, *opsp;   // opsp: operator blocks (LDS or global)
     // chain arrays (global)
     double *tau;
@@ -275,6 +276,12 @@ struct Ctx {
         int ca = op_ca(c);
         if (adjoint) ca = 3 - ca;
         return cn[((s * M.nb + op_orb(c)) * 2 + op_spin(c)) * 2 + (ca - 1)];
+    }
+    DEV int4 trans(int s, int c, bool adjoint) const
+    {
+        int ca = op_ca(c);
+        if (adjoint) ca = 3 - ca;
+        return tr[((s * M.nb + op_orb(c)) * 2 + op_spin(c)) * 2 + (ca - 1)];
     }
     DEV const double *opblock(int s, int c, bool adjoint) const
     {
@@ -424,6 +431,16 @@ DEV double prop_lane(const Ctx &c, int s, double dt, double &lg)
     const int k = lane_id();
     const double ek = k < ns ? E[k] : e0;
     return exp(-(ek - e0) * dt);   // exactly 1 beyond the block and for 1-state blocks
+}
+
+// prop_lane for a block given by its energies E and size ns
+DEV double prop_e(const Ctx &c, const double *E, int ns, double dt, double &lg)
+{
+    const double e0 = E[0];
+    lg += -(e0 - c.M.egs) * dt;
+    const int k = lane_id();
+    const double ek = k < ns ? E[k] : e0;
+    return exp(-(ek - e0) * dt);
 }
 
 // Thresholds outside which a state matrix is renormalised during a walk; in
@@ -592,32 +609,38 @@ DEV bool ket_walk(Ctx &c, double *&K, double *&Kw, int &s, double &lg, int no, d
     double t = t0;
     Work w;
     if constexpr (NM <= 16) {
-        // the next operator's time, code, target block and propagator factors
-        // are read and computed before the current step's barrier
+        // One LDS round trip per operator: the transition record of the next
+        // operator, its target block's energies, the current block entries and
+        // state are read together; operator codes and times two ahead.
         const LaneSlots<NM> ls(no);
         double tj = n > 0 ? otau[0] : t1;
         int cj = n > 0 ? ocode[0] : 0;
-        double d = prop_lane(c, s, tj - t, lg);
+        double tn = n > 1 ? otau[1] : t1;
+        int cn = n > 1 ? ocode[1] : 0;
+        int ns = c.sz[s];
+        double d = prop_e(c, c.ev + c.off[s], ns, tj - t, lg);
+        int4 tr = n > 0 ? c.trans(s, cj, false) : make_int4(0, 0, 0, 0);
         for (int j = 0; j < n; ++j) {
-            const int ts = c.conn(s, cj, false);
-            if (ts < 0) return w.commit(c), false;
-            const int ns = c.sz[s], nt = c.sz[ts];
-            const double *P = c.opblock(s, cj, false);
-            const double tn = j + 1 < n ? otau[j + 1] : t1;
-            const int cn = j + 1 < n ? ocode[j + 1] : 0;
-            const double dn = prop_lane(c, ts, tn - tj, lg);
-            w.op(nt, ns, no);
-            apply_small<NM>(P, K, Kw, nt, ns, no, d, ls, lg);
+            if (tr.x < 0) return w.commit(c), false;
+            const double tnn = j + 2 < n ? otau[j + 2] : t1;
+            const int cnn = j + 2 < n ? ocode[j + 2] : 0;
+            const int4 trn = j + 1 < n ? c.trans(tr.x, cn, false) : make_int4(0, 0, 0, 0);
+            const double dn = prop_e(c, c.ev + tr.w, tr.y, tn - tj, lg);
+            w.op(tr.y, ns, no);
+            apply_small<NM>(c.opsp + tr.z, K, Kw, tr.y, ns, no, d, ls, lg);
             double *x = K;
             K = Kw;
             Kw = x;
-            s = ts;
-            t = tj;
-            tj = tn;
-            cj = cn;
+            s = tr.x;
+            ns = tr.y;
+            tr = trn;
             d = dn;
+            tj = tn;
+            tn = tnn;
+            cj = cn;
+            cn = cnn;
         }
-        finish_small<NM>(K, c.sz[s], no, d, ls, lg);
+        finish_small<NM>(K, ns, no, d, ls, lg);
     } else {
         for (int j = 0; j < n; ++j) {
             const double tj = otau[j];
@@ -652,26 +675,31 @@ DEV void bra_walk(Ctx &c, double *&B, double *&Bw, int &s, double &lg, int no, d
         const LaneSlots<NM> ls(no);
         double tj = n > 0 ? otau[n - 1] : t0;
         int cj = n > 0 ? ocode[n - 1] : 0;
-        double d = prop_lane(c, s, t - tj, lg);
+        double tn = n > 1 ? otau[n - 2] : t0;
+        int cn = n > 1 ? ocode[n - 2] : 0;
+        int ns = c.sz[s];
+        double d = prop_e(c, c.ev + c.off[s], ns, t - tj, lg);
+        int4 tr = n > 0 ? c.trans(s, cj, true) : make_int4(0, 0, 0, 0);
         for (int j = n - 1; j >= 0; --j) {
-            const int ts = c.conn(s, cj, true);
-            const int ns = c.sz[s], nt = c.sz[ts];
-            const double *P = c.opblock(s, cj, true);
-            const double tn = j > 0 ? otau[j - 1] : t0;
-            const int cn = j > 0 ? ocode[j - 1] : 0;
-            const double dn = prop_lane(c, ts, tj - tn, lg);
-            w.op(nt, ns, no);
-            apply_small<NM>(P, B, Bw, nt, ns, no, d, ls, lg);
+            const double tnn = j >= 2 ? otau[j - 2] : t0;
+            const int cnn = j >= 2 ? ocode[j - 2] : 0;
+            const int4 trn = j >= 1 ? c.trans(tr.x, cn, true) : make_int4(0, 0, 0, 0);
+            const double dn = prop_e(c, c.ev + tr.w, tr.y, tj - tn, lg);
+            w.op(tr.y, ns, no);
+            apply_small<NM>(c.opsp + tr.z, B, Bw, tr.y, ns, no, d, ls, lg);
             double *x = B;
             B = Bw;
             Bw = x;
-            s = ts;
-            t = tj;
-            tj = tn;
-            cj = cn;
+            s = tr.x;
+            ns = tr.y;
+            tr = trn;
             d = dn;
+            tj = tn;
+            tn = tnn;
+            cj = cn;
+            cn = cnn;
         }
-        finish_small<NM>(B, c.sz[s], no, d, ls, lg);
+        finish_small<NM>(B, ns, no, d, ls, lg);
     } else {
         for (int j = n - 1; j >= 0; --j) {
             const double tj = otau[j];
@@ -2110,6 +2138,12 @@ DEV Ctx make_ctx(const DevModel &M, const ChainLayout &L, char *base, int chain,
         oo[i] = (int)M.opoff[i];
     }
     for (int i = lane_id(); i < M.nstates; i += WAVE) ev[i] = M.evals[i];
+    int4 *trt = reinterpret_cast<int4 *>(lds + L.l_tr);
+    for (int i = lane_id(); i < ncn; i += WAVE) {
+        const int ts = M.connect[i];
+        trt[i] = make_int4(ts, ts >= 0 ? M.sizes[ts] : 0, (int)M.opoff[i], ts >= 0 ? M.offsets[ts] : 0);
+    }
+    c.tr = trt;
     // operator blocks: always LDS for blocks up to 4 states (known at compile
     // time, so the block reads are LDS instructions); LDS or HBM above
     if constexpr (NM <= 4) {

@@ -61,7 +61,7 @@ struct ChainLayout {
     long kmat;            // max(kcap*kcap, nmax*nmax)
     // dynamic LDS carve-up (byte offsets; l_ops / l_k = -1: kept in global memory)
     long l_hdr, l_mt, l_sz, l_off, l_cn, l_oo, l_ev, l_ops, l_wtau, l_wcode, l_ttau, l_tcode;
-    long l_dvec, l_q, l_r, l_col, l_row, l_pl, l_k, l_legc, lds_bytes;
+    long l_dvec, l_q, l_r, l_col, l_row, l_pl, l_k, l_legc, l_tr, lds_bytes;
     int lcap_lds;         // operators each LDS list holds (window list, trial list)
 };
 
