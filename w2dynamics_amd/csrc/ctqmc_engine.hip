@@ -22,6 +22,7 @@ struct w2d_ctqmc {
     w2d::ChainLayout L{};
     int nchains = 0, nm = 0, device = 0, kcap = 0, ntau = 0, nleg = 0, maxhisto = 0;
     int lds_max = 64 * 1024;   // LDS bytes a workgroup may allocate on this device
+    bool moved = false;        // chains have taken steps since create
     int nwin_alloc = 0;
     uint32_t seed0 = 0;
     double beta = 0.0, taudiff_param = -1.0;
@@ -143,7 +144,10 @@ int layout_and_alloc(w2d_ctqmc *h, int nbound)
         q = (q + bytes + 15) / 16 * 16;
         return at;
     };
-    const int lc = std::min(L.cap + 2, 1024);
+    // list capacity: the window's share of the operator capacity with a margin
+    const int nwin = std::max(nbound - 2, 1);
+    const int lc = (int)std::min<long>(std::min<long>(L.cap + 2, 1024),
+                                       std::max<long>(64, 2L * L.cap / (nwin + 1) + 32));
     L.lcap_lds = lc;
     L.l_hdr = lds(sizeof(ChainHdr));
     L.l_mt = lds(624 * 4);
@@ -166,6 +170,17 @@ int layout_and_alloc(w2d_ctqmc *h, int nbound)
     L.l_pl = lds((long)(2 * nm + 2) * 8);
     L.l_k = nm <= 16 ? lds(2L * nm * nm * 8) : -1;
     L.l_legc = lds(2L * std::max(h->nleg, 1) * 8);
+    L.sc = nm <= 4 ? std::min(64, lc - 2) : 0;
+    if (L.sc > 0) {
+        L.l_rk = lds((long)(L.sc + 1) * nm * nm * 8);
+        L.l_lb = lds((long)(L.sc + 1) * nm * nm * 8);
+        L.l_rkx = lds((long)(L.sc + 1) * 8);
+        L.l_lbx = lds((long)(L.sc + 1) * 8);
+        L.l_rks = lds((long)(L.sc + 1) * 4);
+        L.l_lbs = lds((long)(L.sc + 1) * 4);
+    } else {
+        L.l_rk = L.l_lb = L.l_rkx = L.l_lbx = L.l_rks = L.l_lbs = -1;
+    }
     const long opsb = (long)m.ops.size() * 8;
     // blocks of up to 4 states: the kernels read operator blocks from LDS only
     // operator blocks join LDS when the total stays within 64 KB (more would
@@ -302,6 +317,7 @@ int map_err(int e)
 // run nmoves steps per chain in launches of whole measurement intervals
 int run_moves(w2d_ctqmc *h, long long nmoves, long long ncorr)
 {
+    h->moved = true;
     long long per = ncorr > 0 ? ncorr * std::max<long long>(1, 4000 / ncorr) : 4000;
     HIPCHK(hipEventRecord(h->ev0, h->stream));
     h->last_launches = 0;
@@ -443,9 +459,12 @@ int w2d_ctqmc_set_taudiffmax(w2d_ctqmc *h, double tdm)
     if (!h) return W2D_ERR_ARG;
     if (!(tdm > 0.0)) return W2D_OK;   // set_taudiffmax with a non-positive value keeps one window
     int nb = nbound_for(tdm, h->beta);
-    if (nb > h->nwin_alloc) {
-        // re-create the chains' memory; the chains are still in their initial
-        // state, which is re-derived from the seeds
+    if (nb != h->nwin_alloc) {
+        // the window count sizes the boundary caches and the LDS operator
+        // lists: re-create the chains' memory.  Only before the chains have
+        // moved (as the reference's driver calls it), since their initial
+        // state is re-derived from the seeds.
+        if (h->moved) return W2D_ERR_STATE;
         int st = layout_and_alloc(h, nb);
         if (st) return st;
         if ((st = init_chains(h))) return st;
@@ -577,6 +596,7 @@ int w2d_ctqmc_get_model(const w2d_ctqmc *h, int *sizes, int *offsets, double *ev
 
 int w2d_ctqmc_moves_log(w2d_ctqmc *h, int chain, int64_t n, w2d_move_record *rec)
 {
+    if (h) h->moved = true;
     if (!h || chain < 0 || chain >= h->nchains || n < 0 || (!rec && n > 0)) return W2D_ERR_ARG;
     const long long per = 4000;
     w2d_move_record *d = nullptr;
@@ -631,6 +651,7 @@ int w2d_ctqmc_set_config(w2d_ctqmc *h, int chain, int n, const double *tau, cons
     if (!h || chain < 0 || chain >= h->nchains || n < 0 || n > h->L.cap || outer_sst < 0 ||
         outer_sst >= h->model.nsst)
         return W2D_ERR_ARG;
+    h->moved = true;
     std::vector<int> cnt(2 * h->model.nbands * 2, 0);
     std::vector<int> code(n);
     for (int i = 0; i < n; ++i) {

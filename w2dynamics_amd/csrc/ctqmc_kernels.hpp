@@ -266,6 +266,10 @@ struct Ctx {
     int *tcode;
     double *K0, *K1;     // trace work matrices (LDS for small blocks)
     double *dvec, *Q, *R, *colb, *rowb, *pl;
+    // window prefix kets rk[j] (after the first j window operators) and suffix
+    // bras lb[m] (after the last m), with their logs and superstates (LDS)
+    double *rk, *lb, *rkx, *lbx;
+    int *rks, *lbs;
     const double *legc;  // Legendre recursion coefficients (LDS)
     int nm;              // padded block size used for matrix strides
     DEV Ctx(const DevModel &m, const ChainLayout &l, char *b, ChainHdr &hd) : M(m), L(l), base(b), h(hd) {}
@@ -371,6 +375,8 @@ DEV bool ensure_window(Ctx &c)
     c.h.wia = ia;
     c.h.wib = ib;
     c.h.wvalid = 1;
+    c.h.rk_n = 0;
+    c.h.lb_n = 0;
     wsync();
     return true;
 }
@@ -797,17 +803,27 @@ DEV void log_add(double &al, double &as, double bl, double bs)
 // column; part traces to c.pl, total in (tl, ts).
 DEV void contract(Ctx &c, const double *K, const double *B, int s, int no, double lg, double &tl, double &ts)
 {
+    // every column carries the same scale exp(lg), so the total is the plain
+    // sum of the columns (the log-space sum of addLogTr, Trace.F90:286-298, of
+    // the same numbers)
     const int ns = c.sz[s];
+    double part = 0.0;
     for (int i = lane_id(); i < no; i += WAVE) {
         double acc = 0.0;
         for (int r = 0; r < ns; ++r) acc = fma(K[r * no + i], B[r * no + i], acc);
         c.pl[2 * i] = lg + log(fabs(acc));
         c.pl[2 * i + 1] = copysign(1.0, acc);
+        part += acc;
+    }
+    const double S = wave_sum(part);
+    if (S == 0.0) {
+        tl = -DBL_MAX;
+        ts = 0.0;
+    } else {
+        tl = lg + log(fabs(S));
+        ts = copysign(1.0, S);
     }
     wsync();
-    tl = -DBL_MAX;
-    ts = 0.0;
-    for (int i = 0; i < no; ++i) log_add(tl, ts, c.pl[2 * i], c.pl[2 * i + 1]);
 }
 
 // Full trace of the current configuration (global moves, tau shifts,
@@ -881,6 +897,8 @@ DEV void ensure_cache(Ctx &c)
 {
     const int p = c.h.nwin > 1 ? c.h.wpos : 0;
     if (c.h.cache_pos == p) return;
+    c.h.rk_n = 0;
+    c.h.lb_n = 0;
     if (c.h.cache_pos < 0 || c.h.cache_pos > p) {
         rebuild_cache<NM>(c);
         return;
@@ -919,6 +937,110 @@ DEV bool window_trace(Ctx &c, int nl, double &tl, double &ts)
     if (!ket_walk<NM>(c, K, Kw, s, lg, no, ta, c.ttau, c.tcode, nl, tb)) return false;
     if (s != c.bndsst[top]) return false;
     contract(c, K, c.bnd + (long)top * c.nm * c.nm, s, no, lg + c.bndlog[top], tl, ts);
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// Window segment caches (blocks of up to 4 states).  rk[j] is the ket after the
+// window's first j operators, at the time of operator j-1 (rk[0] = the cached
+// ket at the window's lower edge); lb[m] is the bra after the window's last m
+// operators, at the time of operator nw-m (lb[0] = the cached bra at the upper
+// edge).  A proposal then walks only the operators between its changes: an
+// insertion at window positions q1 <= q2 walks the two new operators and the
+// q2-q1 between them, a removal of w1 < w2 the w2-w1-1 between them.  The
+// caches are extended on demand and cut back to their unchanged parts after
+// an accepted move; both restart when the window list is reloaded or the
+// boundary caches change.
+// ---------------------------------------------------------------------------
+template <int NM>
+DEV void rk_extend(Ctx &c, int to)
+{
+    constexpr int S2 = NM * NM;
+    const int no = c.sz[c.h.outer];
+    const LaneSlots<NM> ls(no);
+    const double ta = bound_tau(c.h.nwin > 1 ? c.h.wpos : 0, c.h.nwin, c.M.beta);
+    int j = c.h.rk_n - 1;
+    if (j < 0) {
+        copy_mat(c.rk, c.kst, c.sz[c.h.kst_sst] * no);
+        c.rkx[0] = c.h.kst_log;
+        c.rks[0] = c.h.kst_sst;
+        j = 0;
+    }
+    if (j < to) {
+        Work w;
+        int s = c.rks[j];
+        double lg = c.rkx[j];
+        for (; j < to; ++j) {
+            const double tp = j > 0 ? c.wtau[j - 1] : ta, tj = c.wtau[j];
+            const int4 tr = c.trans(s, c.wcode[j], false);
+            const int ns = c.sz[s];
+            const double d = prop_e(c, c.ev + c.off[s], ns, tj - tp, lg);
+            w.op(tr.y, ns, no);
+            apply_small<NM>(c.opsp + tr.z, c.rk + (long)j * S2, c.rk + (long)(j + 1) * S2, tr.y, ns, no, d, ls, lg);
+            s = tr.x;
+            c.rkx[j + 1] = lg;
+            c.rks[j + 1] = s;
+        }
+        w.commit(c);
+    }
+    c.h.rk_n = max(c.h.rk_n, to + 1);
+    wsync();
+}
+
+template <int NM>
+DEV void lb_extend(Ctx &c, int nw, int to)
+{
+    constexpr int S2 = NM * NM;
+    const int no = c.sz[c.h.outer];
+    const LaneSlots<NM> ls(no);
+    const int top = (c.h.nwin > 1 ? c.h.wpos : 0) + 2;
+    const double tb = bound_tau(top, c.h.nwin, c.M.beta);
+    int m = c.h.lb_n - 1;
+    if (m < 0) {
+        copy_mat(c.lb, c.bnd + (long)top * c.nm * c.nm, c.sz[c.bndsst[top]] * no);
+        c.lbx[0] = c.bndlog[top];
+        c.lbs[0] = c.bndsst[top];
+        m = 0;
+    }
+    if (m < to) {
+        Work w;
+        int s = c.lbs[m];
+        double lg = c.lbx[m];
+        for (; m < to; ++m) {
+            const int j = nw - 1 - m;
+            const double tu = m > 0 ? c.wtau[j + 1] : tb, tj = c.wtau[j];
+            const int4 tr = c.trans(s, c.wcode[j], true);
+            const int ns = c.sz[s];
+            const double d = prop_e(c, c.ev + c.off[s], ns, tu - tj, lg);
+            w.op(tr.y, ns, no);
+            apply_small<NM>(c.opsp + tr.z, c.lb + (long)m * S2, c.lb + (long)(m + 1) * S2, tr.y, ns, no, d, ls, lg);
+            s = tr.x;
+            c.lbx[m + 1] = lg;
+            c.lbs[m + 1] = s;
+        }
+        w.commit(c);
+    }
+    c.h.lb_n = max(c.h.lb_n, to + 1);
+    wsync();
+}
+
+// Trace of the window with the trial list's entries [a, a + n) replacing the
+// window's operators between rk[j0] and lb[m0]; t0 / t1 are the times of
+// those two states.  Returns false on a quantum-number violation.
+template <int NM>
+DEV bool segment_trace(Ctx &c, int nw, int j0, int m0, int a, int n, double t0, double t1, double &tl, double &ts)
+{
+    constexpr int S2 = NM * NM;
+    const int no = c.sz[c.h.outer];
+    if (c.h.rk_n <= j0) rk_extend<NM>(c, j0);
+    if (c.h.lb_n <= m0) lb_extend<NM>(c, nw, m0);
+    int s = c.rks[j0];
+    double lg = c.rkx[j0];
+    double *K = c.K0, *Kw = c.K1;
+    copy_mat(K, c.rk + (long)j0 * S2, c.sz[s] * no);
+    if (!ket_walk<NM>(c, K, Kw, s, lg, no, t0, c.ttau + a, c.tcode + a, n, t1)) return false;
+    if (s != c.lbs[m0]) return false;
+    contract(c, K, c.lb + (long)m0 * S2, s, no, lg + c.lbx[m0], tl, ts);
     return true;
 }
 
@@ -1307,7 +1429,16 @@ DEV int step_add(Ctx &c)
     wsync();
     const int nl = nw + 2;
     double tl, ts;
-    if (!window_trace<NM>(c, nl, tl, ts)) return c.h.qn_rej++, OUT_QN;
+    const bool seg = NM <= 4 && nw <= c.L.sc;
+    if (seg) {
+        const int p = c.h.nwin > 1 ? c.h.wpos : 0;
+        const double t0 = q1 > 0 ? c.wtau[q1 - 1] : bound_tau(p, c.h.nwin, c.M.beta);
+        const double t1e = q2 < nw ? c.wtau[q2] : bound_tau(p + 2, c.h.nwin, c.M.beta);
+        if (!segment_trace<NM>(c, nw, q1, nw - q2, q1, q2 - q1 + 2, t0, t1e, tl, ts))
+            return c.h.qn_rej++, OUT_QN;
+    } else if (!window_trace<NM>(c, nl, tl, ts)) {
+        return c.h.qn_rej++, OUT_QN;
+    }
     // det ratio, get_LogDetRatPairAdd (Trace.F90:2775-2832): Q (annihilators) and
     // R (creators) placed by physical slot; FPos = 1 + earlier operators of the type
     const long fo = (long)f * c.L.kcap;
@@ -1340,6 +1471,9 @@ DEV int step_add(Ctx &c)
     c.h.alg_flops += 6.0 * k * k;
     c.h.alg_bytes += 8.0 * k * k + 8.0 * (k + 1) * (k + 1);
     commit_window(c, nl);
+    // segment caches: the first q1 and the last nw - q2 window operators are unchanged
+    c.h.rk_n = seg ? min(c.h.rk_n, q1 + 1) : 0;
+    c.h.lb_n = seg ? min(c.h.lb_n, nw - q2 + 1) : 0;
     double *fcf = c.fc + fo, *faf = c.fa + fo;
     int *fcsf = c.fcs + fo, *fasf = c.fas + fo;
     block_move(fcf, fp_c - 1, k, 1);
@@ -1442,7 +1576,16 @@ DEV int step_rem(Ctx &c)
     }
     wsync();
     double tl, ts;
-    if (!window_trace<NM>(c, nl, tl, ts)) return c.h.qn_rej++, OUT_QN;
+    const bool seg = NM <= 4 && nw <= c.L.sc;
+    if (seg) {
+        const int p = c.h.nwin > 1 ? c.h.wpos : 0;
+        const double t0 = w1 > 0 ? c.wtau[w1 - 1] : bound_tau(p, c.h.nwin, c.M.beta);
+        const double t1e = w2 + 1 < nw ? c.wtau[w2 + 1] : bound_tau(p + 2, c.h.nwin, c.M.beta);
+        if (!segment_trace<NM>(c, nw, w1, nw - 1 - w2, w1, w2 - w1 - 1, t0, t1e, tl, ts))
+            return c.h.qn_rej++, OUT_QN;
+    } else if (!window_trace<NM>(c, nl, tl, ts)) {
+        return c.h.qn_rej++, OUT_QN;
+    }
     const long fo = (long)f * c.L.kcap;
     const int fp_c = count_below_par(c.fc + fo, k, tc) + 1, fp_a = count_below_par(c.fa + fo, k, ta) + 1;
     const int sc = c.fcs[fo + fp_c - 1], sa = c.fas[fo + fp_a - 1];
@@ -1471,6 +1614,9 @@ DEV int step_rem(Ctx &c)
     }
     wsync();
     commit_window(c, nl);
+    // segment caches: the first w1 and the last nw - 1 - w2 window operators are unchanged
+    c.h.rk_n = seg ? min(c.h.rk_n, w1 + 1) : 0;
+    c.h.lb_n = seg ? min(c.h.lb_n, nw - w2) : 0;
     block_move(c.fc + fo, fp_c, k, -1);
     block_move(fcsf, fp_c, k, -1);
     block_move(c.fa + fo, fp_a, k, -1);
@@ -2108,6 +2254,14 @@ DEV Ctx make_ctx(const DevModel &M, const ChainLayout &L, char *base, int chain,
     c.colb = reinterpret_cast<double *>(lds + L.l_col);
     c.rowb = reinterpret_cast<double *>(lds + L.l_row);
     c.pl = reinterpret_cast<double *>(lds + L.l_pl);
+    if constexpr (NM <= 4) {
+        c.rk = reinterpret_cast<double *>(lds + L.l_rk);
+        c.lb = reinterpret_cast<double *>(lds + L.l_lb);
+        c.rkx = reinterpret_cast<double *>(lds + L.l_rkx);
+        c.lbx = reinterpret_cast<double *>(lds + L.l_lbx);
+        c.rks = reinterpret_cast<int *>(lds + L.l_rks);
+        c.lbs = reinterpret_cast<int *>(lds + L.l_lbs);
+    }
     {
         double *lc = reinterpret_cast<double *>(lds + L.l_legc);
         for (int i = lane_id(); i < 2 * M.nleg; i += WAVE) lc[i] = M.legc[i];
