@@ -1254,46 +1254,58 @@ DEV bool lu_inplace(double *A, int k, int ld, double &dl, double &ds, int *piv)
         dl += log(fabs(d));
         for (int r = col + 1 + l; r < k; r += WAVE) A[r + (long)col * ld] /= d;
         wsync();
-        for (int j = col + 1; j < k; ++j) {
-            double u = A[col + (long)j * ld];
-            for (int r = col + 1 + l; r < k; r += WAVE) A[r + (long)j * ld] -= A[r + (long)col * ld] * u;
+        // trailing update, lanes over its elements
+        const int m = k - col - 1;
+        for (int idx = l; idx < m * m; idx += WAVE) {
+            const int r = col + 1 + idx % m, j = col + 1 + idx / m;
+            A[r + (long)j * ld] -= A[r + (long)col * ld] * A[col + (long)j * ld];
         }
         wsync();
     }
     return true;
 }
 
-// Inverse from an in-place LU into out (k x k, ld)
-DEV void lu_inverse(const double *LU, int k, int ld, const int *piv, double *out)
+// Inverse from an in-place LU (row swaps piv) into out (k x k, leading
+// dimension ldo): out = U^-1 L^-1 P, by substitution on all columns at once,
+// lanes over the elements of each step.
+DEV void lu_inverse(const double *LU, int k, int ld, const int *piv, double *out, int ldo)
 {
     const int l = lane_id();
-    for (int col = l; col < k; col += WAVE) {
-        double *x = out + (long)col * ld;
-        for (int i = 0; i < k; ++i) x[i] = (i == col) ? 1.0 : 0.0;
+    for (int idx = l; idx < k * k; idx += WAVE) out[idx % k + (long)(idx / k) * ldo] = 0.0;
+    wsync();
+    // row r of the identity ends in row pos after the swaps
+    for (int r = l; r < k; r += WAVE) {
+        int pos = r;
         for (int i = 0; i < k; ++i) {
-            int p = piv[i];
-            double t = x[i];
-            x[i] = x[p];
-            x[p] = t;
+            const int p = piv[i];
+            pos = pos == i ? p : (pos == p ? i : pos);
         }
-        for (int i = 0; i < k; ++i) {
-            double xi = x[i];
-            for (int r = i + 1; r < k; ++r) x[r] -= LU[r + (long)i * ld] * xi;
-        }
-        for (int i = k - 1; i >= 0; --i) {
-            double xi = x[i] / LU[i + (long)i * ld];
-            x[i] = xi;
-            for (int r = 0; r < i; ++r) x[r] -= LU[r + (long)i * ld] * xi;
-        }
+        out[pos + (long)r * ldo] = 1.0;
     }
     wsync();
+    for (int i = 0; i + 1 < k; ++i) {
+        const int m = k - i - 1;
+        for (int idx = l; idx < m * k; idx += WAVE) {
+            const int r = i + 1 + idx % m, cc = idx / m;
+            out[r + (long)cc * ldo] -= LU[r + (long)i * ld] * out[i + (long)cc * ldo];
+        }
+        wsync();
+    }
+    for (int i = k - 1; i >= 0; --i) {
+        for (int cc = l; cc < k; cc += WAVE) out[i + (long)cc * ldo] /= LU[i + (long)i * ld];
+        wsync();
+        for (int idx = l; idx < i * k; idx += WAVE) {
+            const int r = idx % i, cc = idx / i;
+            out[r + (long)cc * ldo] -= LU[r + (long)i * ld] * out[i + (long)cc * ldo];
+        }
+        wsync();
+    }
 }
 
 // Delta_f(row = annihilator rank, col = creator rank) = F(tau_a - tau_c)
 // (get_InvFullHybr, Trace.F90:2279-2333) from the per-flavour lists
-DEV void build_delta(Ctx &c, int f, int k, double *D, const double *fcl, const double *fal)
+DEV void build_delta(Ctx &c, int f, int k, double *D, int ld, const double *fcl, const double *fal)
 {
-    const int ld = c.L.kcap;
     const double *ct = fcl + (long)f * c.L.kcap, *at = fal + (long)f * c.L.kcap;
     for (int idx = lane_id(); idx < k * k; idx += WAVE) {
         int r = idx % k, col = idx / k;
@@ -1785,13 +1797,26 @@ DEV int step_glob(Ctx &c)
     int *piv = reinterpret_cast<int *>(c.R);
     double dnl = 0.0, dns = 1.0, dol = 0.0, dos = 1.0;
     const int ld = c.L.kcap;
+    // LDS work area for the k x k blocks (blocks up to 4 states: the window
+    // segment caches, restarted after this move); HBM scratch otherwise
+    double *wa = nullptr;
+    long wcap = 0;
+    if constexpr (NM <= 4) {
+        wa = c.rk;
+        wcap = 2L * (c.L.sc + 1) * NM * NM;
+        c.h.rk_n = 0;
+        c.h.lb_n = 0;
+    }
     bool sing = false;
     for (int f = 0; f < nf; ++f) {
         const int k = tkf[f];
         if (k > 0) {
-            build_delta(c, f, k, c.scratch, tfc, tfa);
+            const bool inl = (long)k * k <= wcap;
+            double *A = inl ? wa : c.scratch;
+            const int lda = inl ? k : ld;
+            build_delta(c, f, k, A, lda, tfc, tfa);
             double l, s;
-            if (!lu_inplace(c.scratch, k, ld, l, s, piv)) sing = true;
+            if (!lu_inplace(A, k, lda, l, s, piv)) sing = true;
             dnl += l;
             dns *= s;
         }
@@ -1811,10 +1836,21 @@ DEV int step_glob(Ctx &c)
         for (int f = 0; f < nf; ++f) {
             const int k = c.h.kf[f];
             if (k == 0) continue;
-            build_delta(c, f, k, c.scratch, c.fc, c.fa);
             double l, s;
-            lu_inplace(c.scratch, k, ld, l, s, piv);
-            lu_inverse(c.scratch, k, ld, piv, c.mi(f));
+            if (2L * k * k <= wcap) {
+                double *A = wa, *X = wa + (long)k * k;
+                build_delta(c, f, k, A, k, c.fc, c.fa);
+                lu_inplace(A, k, k, l, s, piv);
+                lu_inverse(A, k, k, piv, X, k);
+                double *Mf = c.mi(f);
+                for (int idx = lane_id(); idx < k * k; idx += WAVE)
+                    Mf[idx % k + (long)(idx / k) * ld] = X[idx];
+                wsync();
+            } else {
+                build_delta(c, f, k, c.scratch, ld, c.fc, c.fa);
+                lu_inplace(c.scratch, k, ld, l, s, piv);
+                lu_inverse(c.scratch, k, ld, piv, c.mi(f), ld);
+            }
         }
         c.h.det_log = dnl;
         c.h.det_sign = dns;
@@ -2076,20 +2112,30 @@ DEV double row_reduce_scatter16(double (&v)[16])
 // accumulators (up to 128 orders per pass over the pairs), combined across
 // rows once per flavour.  Kept out of line: inlined into the sweep kernel, the
 // ROCm 7.2 backend emits an invalid V_CMP (src_shared_base operand) for it.
-__device__ __attribute__((noinline)) void meas_gtau(Ctx &c, double *gtau, double *gleg)
+struct GtauArgs {
+    int nf, ntau, nleg, kcap;
+    double beta, cfgsign;
+    const double *legc;
+    const int *kf;
+    const double *fc, *fa, *minv;
+    const int *fcs, *fas;
+    double *gtau, *gleg;
+};
+
+__device__ __attribute__((noinline)) double2 meas_gtau(const GtauArgs a)
 {
-    const DevModel &M = c.M;
-    const int ld = c.L.kcap;
+    const int ld = a.kcap;
     constexpr int LPASS = 128, NCH = LPASS / 16;
-    for (int f = 0; f < M.nf; ++f) {
-        const int k = c.h.kf[f];
-        const double *ctl = c.fc + (long)f * c.L.kcap, *atl = c.fa + (long)f * c.L.kcap;
-        const int *csl = c.fcs + (long)f * c.L.kcap, *asl = c.fas + (long)f * c.L.kcap;
-        const double *Mf = c.mi(f);
-        c.h.alg_flops += (double)k * k * (5.0 * M.nleg + 4.0);
-        c.h.alg_bytes += 8.0 * k * k;
+    double wflops = 0.0, wbytes = 0.0;
+    for (int f = 0; f < a.nf; ++f) {
+        const int k = a.kf[f];
+        const double *ctl = a.fc + (long)f * a.kcap, *atl = a.fa + (long)f * a.kcap;
+        const int *csl = a.fcs + (long)f * a.kcap, *asl = a.fas + (long)f * a.kcap;
+        const double *Mf = a.minv + (long)f * a.kcap * a.kcap;
+        wflops += (double)k * k * (5.0 * a.nleg + 4.0);
+        wbytes += 8.0 * k * k;
         const int npair = k * k;
-        for (int l0 = 0; l0 < M.nleg; l0 += LPASS) {
+        for (int l0 = 0; l0 < a.nleg; l0 += LPASS) {
             double acc[NCH];
 #pragma unroll
             for (int q = 0; q < NCH; ++q) acc[q] = 0.0;
@@ -2102,22 +2148,22 @@ __device__ __attribute__((noinline)) void meas_gtau(Ctx &c, double *gtau, double
                     Mv = Mf[csl[ci] + (long)asl[ai] * ld];
                     int idt;
                     if (ta - tc > 0.0) {
-                        idt = (int)((ta - tc) / M.beta * (double)M.ntau) + 1;
-                        x = 2.0 / M.beta * (ta - tc) - 1.0;
+                        idt = (int)((ta - tc) / a.beta * (double)a.ntau) + 1;
+                        x = 2.0 / a.beta * (ta - tc) - 1.0;
                     } else {
-                        idt = (int)((M.beta + ta - tc) / M.beta * (double)M.ntau) + 1;
-                        x = 2.0 / M.beta * (M.beta + ta - tc) - 1.0;
+                        idt = (int)((a.beta + ta - tc) / a.beta * (double)a.ntau) + 1;
+                        x = 2.0 / a.beta * (a.beta + ta - tc) - 1.0;
                         Mv = -Mv;
                     }
-                    Mv = Mv * c.h.cfgsign;
-                    if (idt > M.ntau) idt = M.ntau;
-                    if (l0 == 0) global_add(&gtau[(long)f * M.ntau + idt - 1], Mv);
+                    Mv = Mv * a.cfgsign;
+                    if (idt > a.ntau) idt = a.ntau;
+                    if (l0 == 0) global_add(&a.gtau[(long)f * a.ntau + idt - 1], Mv);
                 }
                 // Bonnet recursion from order 0 up to the end of this pass
                 double pl = 0.0, plm1 = 0.0;
-                const int lend = min(M.nleg, l0 + LPASS);
+                const int lend = min(a.nleg, l0 + LPASS);
                 for (int l = 0; l < l0; ++l) {
-                    const double plp1 = l == 0 ? 1.0 : (l == 1 ? x : c.legc[2 * (l - 1)] * x * pl - c.legc[2 * (l - 1) + 1] * plm1);
+                    const double plp1 = l == 0 ? 1.0 : (l == 1 ? x : a.legc[2 * (l - 1)] * x * pl - a.legc[2 * (l - 1) + 1] * plm1);
                     plm1 = pl;
                     pl = plp1;
                 }
@@ -2131,7 +2177,7 @@ __device__ __attribute__((noinline)) void meas_gtau(Ctx &c, double *gtau, double
                         const int l = lb + j;
                         double plp1 = 0.0;
                         if (l < lend) {
-                            plp1 = l == 0 ? 1.0 : (l == 1 ? x : c.legc[2 * (l - 1)] * x * pl - c.legc[2 * (l - 1) + 1] * plm1);
+                            plp1 = l == 0 ? 1.0 : (l == 1 ? x : a.legc[2 * (l - 1)] * x * pl - a.legc[2 * (l - 1) + 1] * plm1);
                             plm1 = pl;
                             pl = plp1;
                         }
@@ -2144,16 +2190,17 @@ __device__ __attribute__((noinline)) void meas_gtau(Ctx &c, double *gtau, double
 #pragma unroll
             for (int q = 0; q < NCH; ++q) {
                 const int lb = l0 + 16 * q;
-                if (lb >= M.nleg) break;
+                if (lb >= a.nleg) break;
                 double r = acc[q];
                 r += __shfl_xor(r, 16, WAVE);
                 r += __shfl_xor(r, 32, WAVE);
                 const int l = lb + lane_id();
-                if (lane_id() < 16 && l < M.nleg) gleg[(long)f * M.nleg + l] += r;
+                if (lane_id() < 16 && l < a.nleg) a.gleg[(long)f * a.nleg + l] += r;
             }
         }
     }
     wsync();
+    return make_double2(wflops, wbytes);
 }
 
 // MeasDensityMatrix_beta_half (CTQMC.F90:3088-3151) with update_b2_states_eb
@@ -2367,7 +2414,14 @@ __global__ void __launch_bounds__(WAVE) sweep_kernel(DevModel M, ChainLayout L, 
             c.h.cnt_meas++;
             meas_misc(c, histo, ohist);
             if (M.meas_dm) meas_dm<NM>(c, dm);
-            meas_gtau(c, gtau, gleg);
+            GtauArgs ga;
+            ga.nf = M.nf, ga.ntau = M.ntau, ga.nleg = M.nleg, ga.kcap = L.kcap;
+            ga.beta = M.beta, ga.cfgsign = c.h.cfgsign, ga.legc = c.legc;
+            ga.kf = c.h.kf, ga.fc = c.fc, ga.fa = c.fa, ga.minv = c.minv, ga.fcs = c.fcs, ga.fas = c.fas;
+            ga.gtau = gtau, ga.gleg = gleg;
+            const double2 wk = meas_gtau(ga);
+            c.h.alg_flops += wk.x;
+            c.h.alg_bytes += wk.y;
         }
     }
     save_ctx(c);
@@ -2409,13 +2463,13 @@ __global__ void __launch_bounds__(WAVE) setconfig_kernel(DevModel M, ChainLayout
     for (int f = 0; f < M.nf; ++f) {
         const int k = c.h.kf[f];
         if (k == 0) continue;
-        build_delta(c, f, k, c.scratch, c.fc, c.fa);
+        build_delta(c, f, k, c.scratch, L.kcap, c.fc, c.fa);
         double l, sg;
         if (!lu_inplace(c.scratch, k, L.kcap, l, sg, piv)) {
             c.h.err = ERR_SING;
             break;
         }
-        lu_inverse(c.scratch, k, L.kcap, piv, c.mi(f));
+        lu_inverse(c.scratch, k, L.kcap, piv, c.mi(f), L.kcap);
         dl += l;
         dsg *= sg;
     }
