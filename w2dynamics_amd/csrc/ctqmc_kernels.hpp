@@ -1142,13 +1142,12 @@ DEV double config_sign(Ctx &c)
 // S - R^T M Q for an order-k block (get_DetRatAdd)
 DEV double detrat_add(Ctx &c, const double *Mf, int k, double S)
 {
+    // lanes over the k^2 elements: every load of M is issued at once
     const int ld = c.L.kcap;
     double part = 0.0;
-    for (int i = lane_id(); i < k; i += WAVE) {
-        double v = 0.0;
-#pragma unroll 8
-        for (int j = 0; j < k; ++j) v = fma(Mf[i + (long)j * ld], c.Q[j], v);
-        part = fma(c.R[i], v, part);
+    for (int idx = lane_id(); idx < k * k; idx += WAVE) {
+        const int i = idx % k, j = idx / k;
+        part = fma(c.R[i] * Mf[i + (long)j * ld], c.Q[j], part);
     }
     return S - wave_sum(part);
 }
