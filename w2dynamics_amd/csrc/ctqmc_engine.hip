@@ -161,6 +161,7 @@ int layout_and_alloc(w2d_ctqmc *h, int nbound)
     L.l_wcode = lds((long)lc * 4);
     L.l_ttau = lds((long)lc * 8);
     L.l_tcode = lds((long)lc * 4);
+    L.l_pc = lds((long)lc * 4);
     L.l_dvec = lds((long)nm * 8);
     const long qn = std::max<long>(kc + 2, m.nsst + 1);
     L.l_q = lds(qn * 8);

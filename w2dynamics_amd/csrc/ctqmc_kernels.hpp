@@ -262,6 +262,7 @@ struct Ctx {
     // work buffers
     double *wtau;        // window list (LDS): operators [wia, wib) while h.wvalid
     int *wcode;
+    int *pc;             // removable-pair count of each window operator (LDS, while h.pc_valid)
     double *ttau;        // trial list / staging buffer (LDS), same capacity
     int *tcode;
     double *K0, *K1;     // trace work matrices (LDS for small blocks)
@@ -375,6 +376,7 @@ DEV bool ensure_window(Ctx &c)
     c.h.wia = ia;
     c.h.wib = ib;
     c.h.wvalid = 1;
+    c.h.pc_valid = 0;
     c.h.rk_n = 0;
     c.h.lb_n = 0;
     wsync();
@@ -1482,6 +1484,7 @@ DEV int step_add(Ctx &c)
     c.h.alg_flops += 6.0 * k * k;
     c.h.alg_bytes += 8.0 * k * k + 8.0 * (k + 1) * (k + 1);
     commit_window(c, nl);
+    c.h.pc_valid = 0;
     // segment caches: the first q1 and the last nw - q2 window operators are unchanged
     c.h.rk_n = seg ? min(c.h.rk_n, q1 + 1) : 0;
     c.h.lb_n = seg ? min(c.h.lb_n, nw - q2 + 1) : 0;
@@ -1531,12 +1534,16 @@ DEV int step_rem(Ctx &c)
     const int nw = c.h.wib - c.h.wia;
     ensure_cache<NM>(c);
     // pair counts per window operator (the window holds every removable pair)
-    int np = 0, cnt0 = 0;   // cnt0: this lane's count in the first 64 operators
+    // (kept in LDS until the window list changes)
+    if (!c.h.pc_valid) {
+        for (int j = lane_id(); j < nw; j += WAVE) c.pc[j] = pairs_from(c, c.wtau, c.wcode, nw, j);
+        c.h.pc_valid = 1;
+        wsync();
+    }
+    int np = 0;
     for (int j0 = 0; j0 < nw; j0 += WAVE) {
         const int j = j0 + lane_id();
-        const int cnt = j < nw ? pairs_from(c, c.wtau, c.wcode, nw, j) : 0;
-        if (j0 == 0) cnt0 = cnt;
-        np += wave_isum(cnt);
+        np += wave_isum(j < nw ? c.pc[j] : 0);
     }
     if (np <= 0) return c.h.qn_rej++, OUT_QN;
     int pick = randint(c.rng, 1, np);
@@ -1545,7 +1552,7 @@ DEV int step_rem(Ctx &c)
         int base = 0;
         for (int j0 = 0; j0 < nw && w1 < 0; j0 += WAVE) {
             const int j = j0 + lane_id();
-            const int cnt = j0 == 0 ? cnt0 : (j < nw ? pairs_from(c, c.wtau, c.wcode, nw, j) : 0);
+            const int cnt = j < nw ? c.pc[j] : 0;
             int tot;
             const int ex = wave_excl_scan(cnt, &tot);
             int found = (pick > base + ex && pick <= base + ex + cnt) ? j : -1;
@@ -1625,6 +1632,7 @@ DEV int step_rem(Ctx &c)
     }
     wsync();
     commit_window(c, nl);
+    c.h.pc_valid = 0;
     // segment caches: the first w1 and the last nw - 1 - w2 window operators are unchanged
     c.h.rk_n = seg ? min(c.h.rk_n, w1 + 1) : 0;
     c.h.lb_n = seg ? min(c.h.lb_n, nw - w2) : 0;
@@ -2292,6 +2300,7 @@ DEV Ctx make_ctx(const DevModel &M, const ChainLayout &L, char *base, int chain,
     double *ev = reinterpret_cast<double *>(lds + L.l_ev);
     c.wtau = reinterpret_cast<double *>(lds + L.l_wtau);
     c.wcode = reinterpret_cast<int *>(lds + L.l_wcode);
+    c.pc = reinterpret_cast<int *>(lds + L.l_pc);
     c.ttau = reinterpret_cast<double *>(lds + L.l_ttau);
     c.tcode = reinterpret_cast<int *>(lds + L.l_tcode);
     c.dvec = reinterpret_cast<double *>(lds + L.l_dvec);

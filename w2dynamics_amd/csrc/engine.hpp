@@ -28,7 +28,7 @@ struct DevModel {
 struct ChainHdr {
     int nop, outer, wpos, nwin;
     int err, cache_pos, kst_sst, mti;
-    int wia, wib, wvalid, pad0;    // cached index range of the window's operators
+    int wia, wib, wvalid, pc_valid; // cached index range of the window's operators; pair counts kept
     int rk_n, lb_n;                // valid entries of the window's prefix kets / suffix bras
     double taudiff_max, tauwin_min, tauwin_max, kst_log;
     double trace_log, trace_sign, det_log, det_sign, cfgsign;
@@ -62,7 +62,7 @@ struct ChainLayout {
     long kmat;            // max(kcap*kcap, nmax*nmax)
     // dynamic LDS carve-up (byte offsets; l_ops / l_k = -1: kept in global memory)
     long l_hdr, l_mt, l_sz, l_off, l_cn, l_oo, l_ev, l_ops, l_wtau, l_wcode, l_ttau, l_tcode;
-    long l_dvec, l_q, l_r, l_col, l_row, l_pl, l_k, l_legc, l_tr, lds_bytes;
+    long l_dvec, l_q, l_r, l_col, l_row, l_pl, l_k, l_legc, l_tr, l_pc, lds_bytes;
     int lcap_lds;         // operators each LDS list holds (window list, trial list)
     // prefix kets / suffix bras of the window's operators (blocks up to 4 states)
     int sc;               // window operators they cover (0: not kept)
