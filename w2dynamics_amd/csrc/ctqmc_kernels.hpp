@@ -564,29 +564,51 @@ DEV void finish_small(double *K, int ns, int no, double d, const LaneSlots<NM> &
     wsync();
 }
 
-// One operator, blocks above 16 states: LDS propagator vector, rescaled to a
-// maximum modulus of one.
-template <int NM>
-DEV void apply_op(Ctx &c, const double *P, const double *Kin, double *Kout, int nt, int ns, int no,
-                  double &lg)
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+// One operator on blocks above 16 states with the FP64 matrix cores:
+// Kout (nt x no, row-major) = P diag(dvec) (P: nt x ns, column-major) Kin
+// (ns x no, row-major), in 16 x 16 output tiles with k-steps of 4.  For
+// v_mfma_f64_16x16x4f64 lane l holds A(l%16, l/16), B(l/16, l%16) and the
+// outputs D(l/16 + 4e, l%16), e = 0..3.  Renormalised (by a power of two)
+// only when the largest entry leaves [2^-64, 2^64].
+DEV void apply_mfma(Ctx &c, const double *P, const double *Kin, double *Kout, int nt, int ns, int no,
+                    double &lg)
 {
-    const int nout = nt * no;
+    const int l = lane_id();
+    const int lr = l & 15, lk = l >> 4;
+    const int tc = (no + 15) >> 4, ntile = ((nt + 15) >> 4) * tc;
     double m = 0.0;
-    for (int idx = lane_id(); idx < nout; idx += WAVE) {
-        int r = idx / no, i = idx - r * no;
-        double acc = 0.0;
-        for (int k = 0; k < ns; ++k) acc = fma(P[r + (long)k * nt] * c.dvec[k], Kin[k * no + i], acc);
-        Kout[idx] = acc;
-        m = fmax(m, fabs(acc));
+    for (int t = 0; t < ntile; ++t) {
+        const int r0 = (t / tc) * 16, i0 = (t % tc) * 16;
+        const int r = r0 + lr, i = i0 + lr;
+        dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+        for (int k0 = 0; k0 < ns; k0 += 4) {
+            const int k = k0 + lk;
+            const double a = (r < nt && k < ns) ? P[r + (long)k * nt] * c.dvec[k] : 0.0;
+            const double b = (k < ns && i < no) ? Kin[(long)k * no + i] : 0.0;
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int ro = r0 + lk + 4 * e;
+            if (ro < nt && i < no) {
+                Kout[(long)ro * no + i] = acc[e];
+                m = fmax(m, fabs(acc[e]));
+            }
+        }
     }
-    m = wave_max(m);
     wsync();
-    if (m > 0.0) {
-        double inv = 1.0 / m;
-        lg += log(m);
-        for (int idx = lane_id(); idx < nout; idx += WAVE) Kout[idx] *= inv;
+    if (__ballot(m >= STATE_TINY) == 0ull || __ballot(m > STATE_HUGE) != 0ull) {
+        const double mm = wave_max(m);
+        if (mm > 0.0) {
+            const int e = ilogb(mm);
+            lg += (double)e * 0.69314718055994530942;
+            for (int idx = l; idx < nt * no; idx += WAVE) Kout[idx] = ldexp(Kout[idx], -e);
+        }
+        wsync();
     }
-    wsync();
 }
 
 // scale rows by dvec and renormalise (end-of-sweep propagation)
@@ -658,7 +680,7 @@ DEV bool ket_walk(Ctx &c, double *&K, double *&Kw, int &s, double &lg, int no, d
             lg += prop_factors(c, s, tj - t);
             const int ns = c.sz[s], nt = c.sz[ts];
             w.op(nt, ns, no);
-            apply_op<NM>(c, c.opblock(s, cj, false), K, Kw, nt, ns, no, lg);
+            apply_mfma(c, c.opblock(s, cj, false), K, Kw, nt, ns, no, lg);
             double *x = K;
             K = Kw;
             Kw = x;
@@ -716,7 +738,7 @@ DEV void bra_walk(Ctx &c, double *&B, double *&Bw, int &s, double &lg, int no, d
             lg += prop_factors(c, s, t - tj);
             const int ns = c.sz[s], nt = c.sz[ts];
             w.op(nt, ns, no);
-            apply_op<NM>(c, c.opblock(s, cj, true), B, Bw, nt, ns, no, lg);
+            apply_mfma(c, c.opblock(s, cj, true), B, Bw, nt, ns, no, lg);
             double *x = B;
             B = Bw;
             Bw = x;
