@@ -571,31 +571,48 @@ typedef double dbl4 __attribute__((ext_vector_type(4)));
 // (ns x no, row-major), in 16 x 16 output tiles with k-steps of 4.  For
 // v_mfma_f64_16x16x4f64 lane l holds A(l%16, l/16), B(l/16, l%16) and the
 // outputs D(l/16 + 4e, l%16), e = 0..3.  Renormalised (by a power of two)
-// only when the largest entry leaves [2^-64, 2^64].
+// only when the largest entry leaves [2^-64, 2^64].  Blocks up to 128 states.
 DEV void apply_mfma(Ctx &c, const double *P, const double *Kin, double *Kout, int nt, int ns, int no,
                     double &lg)
 {
+    // a 16-column strip of the output at a time, all its row tiles in
+    // registers: per k-step one B fragment serves every row tile
+    constexpr int RT = 8;                    // row tiles held (blocks up to 128 states)
     const int l = lane_id();
     const int lr = l & 15, lk = l >> 4;
-    const int tc = (no + 15) >> 4, ntile = ((nt + 15) >> 4) * tc;
+    const int ntr = (nt + 15) >> 4;
     double m = 0.0;
-    for (int t = 0; t < ntile; ++t) {
-        const int r0 = (t / tc) * 16, i0 = (t % tc) * 16;
-        const int r = r0 + lr, i = i0 + lr;
-        dbl4 acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll 4
+    for (int i0 = 0; i0 < no; i0 += 16) {
+        const int i = i0 + lr;
+        dbl4 acc[RT];
+#pragma unroll
+        for (int t = 0; t < RT; ++t) acc[t] = dbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 2
         for (int k0 = 0; k0 < ns; k0 += 4) {
             const int k = k0 + lk;
-            const double a = (r < nt && k < ns) ? P[r + (long)k * nt] * c.dvec[k] : 0.0;
-            const double b = (k < ns && i < no) ? Kin[(long)k * no + i] : 0.0;
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+            const bool kin = k < ns;
+            const double b = (kin && i < no) ? Kin[(long)k * no + i] : 0.0;
+            const double dk = kin ? c.dvec[k] : 0.0;
+#pragma unroll
+            for (int t = 0; t < RT; ++t) {
+                if (t < ntr) {
+                    const int r = t * 16 + lr;
+                    const double a = (r < nt && kin) ? P[r + (long)k * nt] * dk : 0.0;
+                    acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[t], 0, 0, 0);
+                }
+            }
         }
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int ro = r0 + lk + 4 * e;
-            if (ro < nt && i < no) {
-                Kout[(long)ro * no + i] = acc[e];
-                m = fmax(m, fabs(acc[e]));
+        for (int t = 0; t < RT; ++t) {
+            if (t < ntr) {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int ro = t * 16 + lk + 4 * e;
+                    if (ro < nt && i < no) {
+                        Kout[(long)ro * no + i] = acc[t][e];
+                        m = fmax(m, fabs(acc[t][e]));
+                    }
+                }
             }
         }
     }
