@@ -37,7 +37,7 @@ WORKLOADS = {
     "c1": ("anderson_1orb", dict(beta=10.0), 2.0, 1024, 128),
     "c2": ("kanamori_2orb", dict(beta=50.0), 5.0, 1024, 128),
     "c3": ("kanamori_3orb", dict(beta=100.0), 8.0, 4096, 384),
-    "c4": ("coulomb_5orb", dict(beta=40.0), 5.0, 2048, 128),
+    "c4": ("coulomb_5orb", dict(beta=40.0), 5.0, 2048, 64),
 }
 
 

@@ -155,8 +155,10 @@ int layout_and_alloc(w2d_ctqmc *h, int nbound)
     L.l_off = lds((long)m.nsst * 4);
     L.l_cn = lds((long)m.nsst * m.nbands * 4 * 4);
     L.l_oo = lds((long)m.nsst * m.nbands * 4 * 4);
-    L.l_tr = lds((long)m.nsst * m.nbands * 4 * 16);
-    L.l_ev = lds((long)m.nstates * 8);
+    // transition records and energies: LDS for blocks up to 16 states, HBM
+    // above (there the block products dominate and LDS buys chains per CU)
+    L.l_tr = nm <= 16 ? lds((long)m.nsst * m.nbands * 4 * 16) : -1;
+    L.l_ev = nm <= 16 ? lds((long)m.nstates * 8) : -1;
     L.l_wtau = lds((long)lc * 8);
     L.l_wcode = lds((long)lc * 4);
     L.l_ttau = lds((long)lc * 8);
@@ -428,13 +430,18 @@ int w2d_ctqmc_create(const char *params, int nbands, int nftau, const double *u_
     }
     std::vector<int> first_state(m.nsst);
     for (int s = 0; s < m.nsst; ++s) first_state[s] = m.states[m.offsets[s]];
+    std::vector<int4> trans(m.connect.size());
+    for (size_t i = 0; i < m.connect.size(); ++i) {
+        const int ts = m.connect[i];
+        trans[i] = make_int4(ts, ts >= 0 ? m.sizes[ts] : 0, (int)m.opoff[i], ts >= 0 ? m.offsets[ts] : 0);
+    }
     int st = 0;
     if ((st = upload(h.get(), m.sizes, &d.sizes)) || (st = upload(h.get(), m.offsets, &d.offsets)) ||
         (st = upload(h.get(), m.connect, &d.connect)) || (st = upload(h.get(), first_state, &d.first_state)) ||
         (st = upload(h.get(), m.state_sst, &d.state_sst)) || (st = upload(h.get(), m.opoff, &d.opoff)) ||
         (st = upload(h.get(), m.ops, &d.ops)) || (st = upload(h.get(), m.evals, &d.evals)) ||
         (st = upload(h.get(), fdiag, &d.ftau)) || (st = upload(h.get(), dmoff, &d.dmoff)) ||
-        (st = upload(h.get(), legc, &d.legc)))
+        (st = upload(h.get(), legc, &d.legc)) || (st = upload(h.get(), trans, &d.trans)))
         return st;
     d.p_shift = P.real("PercentageTauShiftMove");
     d.p_glob = P.real("PercentageGlobalMove");

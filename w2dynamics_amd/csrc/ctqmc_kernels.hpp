@@ -2385,13 +2385,14 @@ DEV Ctx make_ctx(const DevModel &M, const ChainLayout &L, char *base, int chain,
         cn[i] = M.connect[i];
         oo[i] = (int)M.opoff[i];
     }
-    for (int i = lane_id(); i < M.nstates; i += WAVE) ev[i] = M.evals[i];
-    int4 *trt = reinterpret_cast<int4 *>(lds + L.l_tr);
-    for (int i = lane_id(); i < ncn; i += WAVE) {
-        const int ts = M.connect[i];
-        trt[i] = make_int4(ts, ts >= 0 ? M.sizes[ts] : 0, (int)M.opoff[i], ts >= 0 ? M.offsets[ts] : 0);
+    if constexpr (NM <= 16) {
+        for (int i = lane_id(); i < M.nstates; i += WAVE) ev[i] = M.evals[i];
+        int4 *trt = reinterpret_cast<int4 *>(lds + L.l_tr);
+        for (int i = lane_id(); i < ncn; i += WAVE) trt[i] = M.trans[i];
+        c.tr = trt;
+    } else {
+        c.tr = M.trans;
     }
-    c.tr = trt;
     // operator blocks: always LDS for blocks up to 4 states (known at compile
     // time, so the block reads are LDS instructions); LDS or HBM above
     if constexpr (NM <= 4) {
@@ -2405,7 +2406,9 @@ DEV Ctx make_ctx(const DevModel &M, const ChainLayout &L, char *base, int chain,
     } else {
         c.opsp = M.ops;
     }
-    c.sz = sz, c.off = off, c.cn = cn, c.oo = oo, c.ev = ev;
+    c.sz = sz, c.off = off, c.cn = cn, c.oo = oo;
+    if constexpr (NM <= 16) c.ev = ev;
+    else c.ev = M.evals;
     c.rng.mt = mt;
     c.rng.idx = &hd.mti;
     hd.wvalid = 0;   // the LDS window list does not survive a launch

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <cstdint>
+#include <hip/hip_runtime.h>
 
 namespace w2d {
 
@@ -17,6 +18,7 @@ struct DevModel {
     const double *ops, *evals, *ftau;   // ftau[f][nftau]
     const long *dmoff;                  // block-diagonal density-matrix offsets
     const double *legc;                 // Bonnet coefficients (2l+1)/(l+1), l/(l+1) per l
+    const int4 *trans;                  // per (superstate, operator): target, its size, block offset, its energies offset
     long dmsize;
     long nops;                          // size of ops
     double p_shift, p_glob, p_outer, p_4op;
