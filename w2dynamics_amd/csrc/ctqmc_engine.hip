@@ -173,6 +173,15 @@ int layout_and_alloc(w2d_ctqmc *h, int nbound)
     L.l_pl = lds((long)(2 * nm + 2) * 8);
     L.l_k = nm <= 16 ? lds(2L * nm * nm * 8) : -1;
     L.l_legc = lds(2L * std::max(h->nleg, 1) * 8);
+    // blocks of up to 4 states: quad-walk chunk buffers (16 operators)
+    if (nm <= 4) {
+        L.l_qa = lds(16L * 16 * 8);
+        L.l_qlc = lds(16L * 8);
+        L.l_qrec = lds(16L * 16);
+        L.l_qsst = lds(16L * 4);
+    } else {
+        L.l_qa = L.l_qlc = L.l_qrec = L.l_qsst = -1;
+    }
     L.sc = nm <= 4 ? std::min(64, lc - 2) : 0;
     if (L.sc > 0) {
         L.l_rk = lds((long)(L.sc + 1) * nm * nm * 8);

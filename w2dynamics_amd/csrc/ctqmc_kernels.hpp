@@ -271,6 +271,10 @@ struct Ctx {
     // bras lb[m] (after the last m), with their logs and superstates (LDS)
     double *rk, *lb, *rkx, *lbx;
     int *rks, *lbs;
+    // quad-walk chunk buffers (LDS): entries, path records, targets, log factors
+    double *qa, *qlc;
+    int4 *qrec;
+    int *qsst;
     const double *legc;  // Legendre recursion coefficients (LDS)
     int nm;              // padded block size used for matrix strides
     DEV Ctx(const DevModel &m, const ChainLayout &l, char *b, ChainHdr &hd) : M(m), L(l), base(b), h(hd) {}
@@ -402,16 +406,19 @@ DEV void swap_lists(Ctx &c)
 // bytes = operator block + states in and out.  Accumulated in registers during
 // a walk and added to the chain header once.
 struct Work {
-    double flops = 0.0, bytes = 0.0;
+    long long ntns = 0, nss = 0, nts = 0;
+    int nout = 0;
     DEV void op(int nt, int ns, int no)
     {
-        flops += (double)no * (2.0 * nt * ns + ns);
-        bytes += 8.0 * nt * ns + 8.0 * no * (ns + nt);
+        ntns += nt * ns;
+        nss += ns;
+        nts += nt;
+        nout = no;
     }
     DEV void commit(Ctx &c) const
     {
-        c.h.alg_flops += flops;
-        c.h.alg_bytes += bytes;
+        c.h.alg_flops += (double)nout * (2.0 * (double)ntns + (double)nss);
+        c.h.alg_bytes += 8.0 * (double)ntns + 8.0 * nout * (double)(nss + nts);
     }
 };
 
@@ -564,6 +571,183 @@ DEV void finish_small(double *K, int ns, int no, double d, const LaneSlots<NM> &
     wsync();
 }
 
+// ---------------------------------------------------------------------------
+// Walks on blocks of at most 4 states, in registers.  The (ns x no) state is
+// zero-padded to 4 x 4 and lane l holds its entry (r, i) = ((l / 4) % 4, l % 4);
+// the four 16-lane rows of the wave hold the same matrix.  An operator scales
+// row r by the propagator factor of state r (each lane its own), then forms
+// K'(r, i) = sum_k P(r, k) Kd(k, i): rotations of a 16-lane row by 4, 8 and 12
+// lanes (DPP) bring Kd(k, i) of the other three rows to lane (r, i).  Operator
+// entries and energies are each lane's own LDS reads, issued one operator
+// ahead; no state leaves registers and no barrier is taken per operator.
+// ---------------------------------------------------------------------------
+constexpr int DPP_ROW_ROR4 = 0x124, DPP_ROW_ROR8 = 0x128, DPP_ROW_ROR12 = 0x12C;
+
+struct Quad {
+    int r, i;    // entry held by this lane
+    int k[4];    // row of the entry that the rotation by 4m lanes brings here (k[0] = r)
+    DEV Quad()
+    {
+        const int l = lane_id() & 15;
+        r = l >> 2;
+        i = l & 3;
+        k[0] = r;
+        k[1] = dpp_i<DPP_ROW_ROR4>(r);
+        k[2] = dpp_i<DPP_ROW_ROR8>(r);
+        k[3] = dpp_i<DPP_ROW_ROR12>(r);
+    }
+    // entry of a row-major (ns x no) state, 0 outside
+    DEV double load(const double *K, int ns, int no) const
+    {
+        const bool in = r < ns && i < no;
+        const double x = K[in ? r * no + i : 0];
+        return in ? x : 0.0;
+    }
+    DEV void store(double *K, int ns, int no, double v) const
+    {
+        if (lane_id() < 16 && r < ns && i < no) K[r * no + i] = v;
+    }
+    // P(r, k[m]) of an (nt x ns) column-major block; zeros for a record
+    // without target
+    DEV void entries(const double *ops, int4 tr, int ns, double (&p)[4]) const
+    {
+        const double *P = ops + (tr.x >= 0 ? tr.z : 0);
+        const int nt = tr.x >= 0 ? tr.y : 0;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const bool in = r < nt && k[m] < ns;
+            const double x = P[in ? r + k[m] * nt : 0];
+            p[m] = in ? x : 0.0;
+        }
+    }
+    // exp(-(E_r - E_0) dt) of this lane's row in a block of energies E (1
+    // beyond the block); the common log factor -(E_0 - Egs) dt is returned in lc
+    DEV double prop(const double *E, int ns, double dt, double egs, double &lc) const
+    {
+        const double e0 = E[0];
+        const double er = E[r < ns ? r : 0];
+        lc = -(e0 - egs) * dt;
+        return exp(-(er - e0) * dt);
+    }
+};
+
+// one operator on a quad state with this lane's entries p[m] = A(r, k[m]) of
+// A = P diag(d); power-of-two renormalisation when the largest entry leaves
+// [2^-64, 2^64]
+DEV double quad_apply(double v, const double (&p)[4], double &lg)
+{
+    const double x1 = dpp_d<DPP_ROW_ROR4>(v), x2 = dpp_d<DPP_ROW_ROR8>(v), x3 = dpp_d<DPP_ROW_ROR12>(v);
+    double w = fma(p[1], x1, p[0] * v) + fma(p[3], x3, p[2] * x2);
+    const double m = fabs(w);
+    if (__ballot(m >= STATE_TINY) == 0ull || __ballot(m > STATE_HUGE) != 0ull) {
+        const double mm = wave_max(m);
+        if (mm > 0.0) {
+            const int e = ilogb(mm);
+            lg += (double)e * 0.69314718055994530942;
+            w = ldexp(w, -e);
+        }
+    }
+    return w;
+}
+
+// final propagation (lane factor d) and normalisation to a largest modulus of one
+DEV double quad_finish(double v, double d, double &lg)
+{
+    v *= d;
+    const double m = wave_max(fabs(v));
+    if (m > 0.0) {
+        v *= 1.0 / m;
+        lg += log(m);
+    }
+    return v;
+}
+
+constexpr int QCH = 16;   // operators per chunk of a quad walk
+
+// Walk over the n operators of a list: ascending from entry 0 (ket, times
+// tstart < tau_0 < ... < tend) or, with BRA, descending from entry n-1 with the
+// adjoint blocks (tstart > tau_{n-1} > ... > tend).  In chunks of QCH
+// operators: (1) the superstate path, one transition record per operator
+// (uniform; stops at an operator without target block); (2) the entries
+// A(r, k) = P(r, k) exp(-(E_k - E_0) dt) of each operator's block with its
+// propagator, four operators per pass over the wave, one exp per lane, into
+// LDS; (3) the products in registers.  after(k, v, s, ns, lg) sees the state
+// after the k-th operator walked, at that operator's time.  Returns false at a
+// quantum-number violation; otherwise dlast holds the lane factor of the
+// propagation to tend (its log factor already in lg).
+template <bool BRA, typename F>
+DEV bool quad_walk(Ctx &c, const Quad &q, double &v, int &s, double &lg, int no, double tstart, const double *otau,
+                   const int *ocode, int n, double tend, double &dlast, Work &w, F &&after)
+{
+    const double egs = c.M.egs;
+    const int l = lane_id();
+    // pass (2): lane l computes entry (r, m) = (q.r, q.i), i.e. A(r, k[m])
+    const int km = q.i == 0 ? q.k[0] : q.i == 1 ? q.k[1] : q.i == 2 ? q.k[2] : q.k[3];
+    int ns = c.sz[s], eo = c.off[s];
+    double tlast = tstart;
+    for (int c0 = 0; c0 < n; c0 += QCH) {
+        const int nc = min(QCH, n - c0);
+        // (1) path: record {block offset, target size, source size, source energies offset}
+        for (int kk = 0; kk < nc; ++kk) {
+            const int idx = BRA ? n - 1 - (c0 + kk) : c0 + kk;
+            const int4 rec = c.trans(s, ocode[idx], BRA);
+            if (rec.x < 0) return false;
+            c.qrec[kk] = make_int4(rec.z, rec.y, ns, eo);
+            c.qsst[kk] = rec.x;
+            w.op(rec.y, ns, no);
+            s = rec.x;
+            ns = rec.y;
+            eo = rec.w;
+        }
+        wsync();
+        // (2) entries of the chunk's operators
+        for (int kb = 0; kb < nc; kb += 4) {
+            const int kk = kb + (l >> 4);
+            const bool act = kk < nc;
+            const int kq = act ? kk : kb;
+            const int4 rec = c.qrec[kq];
+            const int j = c0 + kq;
+            const int idx = BRA ? n - 1 - j : j;
+            const double tj = otau[idx];
+            const double tp = j > 0 ? otau[BRA ? idx + 1 : idx - 1] : tstart;
+            const double dt = BRA ? tp - tj : tj - tp;
+            const double e0 = c.ev[rec.w];
+            const bool kin = km < rec.z;
+            const double ek = c.ev[rec.w + (kin ? km : 0)];
+            const bool in = kin && q.r < rec.y;
+            const double pv = c.opsp[in ? rec.x + q.r + km * rec.y : 0];
+            const double a = in ? pv * exp(-(ek - e0) * dt) : 0.0;
+            if (act) {
+                c.qa[kk * 16 + (l & 15)] = a;
+                c.qlc[kk] = -(e0 - egs) * dt;
+            }
+        }
+        wsync();
+        // (3) products
+        const double *pa = c.qa + q.r * 4;
+        double p[4] = {pa[0], pa[1], pa[2], pa[3]};
+        for (int kk = 0; kk < nc; ++kk) {
+            const int kn = kk + 1 < nc ? kk + 1 : kk;
+            const double pn[4] = {pa[kn * 16], pa[kn * 16 + 1], pa[kn * 16 + 2], pa[kn * 16 + 3]};
+            lg += c.qlc[kk];
+            v = quad_apply(v, p, lg);
+            after(c0 + kk, v, c.qsst[kk], c.qrec[kk].y, lg);
+#pragma unroll
+            for (int m = 0; m < 4; ++m) p[m] = pn[m];
+        }
+        tlast = otau[BRA ? n - c0 - nc : c0 + nc - 1];
+        wsync();
+    }
+    double lc;
+    dlast = q.prop(c.ev + eo, ns, BRA ? tlast - tend : tend - tlast, egs, lc);
+    lg += lc;
+    return true;
+}
+
+struct NoAfter {
+    DEV void operator()(int, double, int, int, double) const {}
+};
+
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
 // One operator on blocks above 16 states with the FP64 matrix cores:
@@ -655,7 +839,21 @@ DEV bool ket_walk(Ctx &c, double *&K, double *&Kw, int &s, double &lg, int no, d
 {
     double t = t0;
     Work w;
-    if constexpr (NM <= 16) {
+    if constexpr (NM <= 4) {
+        const Quad q;
+        double v = q.load(K, c.sz[s], no);
+        double d;
+        const bool ok = quad_walk<false>(c, q, v, s, lg, no, t0, otau, ocode, n, t1, d, w, NoAfter());
+        w.commit(c);
+        if (!ok) return false;
+        v = quad_finish(v, d, lg);
+        q.store(Kw, c.sz[s], no, v);
+        double *x = K;
+        K = Kw;
+        Kw = x;
+        wsync();
+        return true;
+    } else if constexpr (NM <= 16) {
         // One LDS round trip per operator: the transition record of the next
         // operator, its target block's energies, the current block entries and
         // state are read together; operator codes and times two ahead.
@@ -718,7 +916,20 @@ DEV void bra_walk(Ctx &c, double *&B, double *&Bw, int &s, double &lg, int no, d
 {
     double t = t1;
     Work w;
-    if constexpr (NM <= 16) {
+    if constexpr (NM <= 4) {
+        const Quad q;
+        double v = q.load(B, c.sz[s], no);
+        double d;
+        quad_walk<true>(c, q, v, s, lg, no, t1, otau, ocode, n, t0, d, w, NoAfter());
+        w.commit(c);
+        v = quad_finish(v, d, lg);
+        q.store(Bw, c.sz[s], no, v);
+        double *x = B;
+        B = Bw;
+        Bw = x;
+        wsync();
+        return;
+    } else if constexpr (NM <= 16) {
         const LaneSlots<NM> ls(no);
         double tj = n > 0 ? otau[n - 1] : t0;
         int cj = n > 0 ? ocode[n - 1] : 0;
@@ -998,7 +1209,6 @@ DEV void rk_extend(Ctx &c, int to)
 {
     constexpr int S2 = NM * NM;
     const int no = c.sz[c.h.outer];
-    const LaneSlots<NM> ls(no);
     const double ta = bound_tau(c.h.nwin > 1 ? c.h.wpos : 0, c.h.nwin, c.M.beta);
     int j = c.h.rk_n - 1;
     if (j < 0) {
@@ -1011,17 +1221,16 @@ DEV void rk_extend(Ctx &c, int to)
         Work w;
         int s = c.rks[j];
         double lg = c.rkx[j];
-        for (; j < to; ++j) {
-            const double tp = j > 0 ? c.wtau[j - 1] : ta, tj = c.wtau[j];
-            const int4 tr = c.trans(s, c.wcode[j], false);
-            const int ns = c.sz[s];
-            const double d = prop_e(c, c.ev + c.off[s], ns, tj - tp, lg);
-            w.op(tr.y, ns, no);
-            apply_small<NM>(c.opsp + tr.z, c.rk + (long)j * S2, c.rk + (long)(j + 1) * S2, tr.y, ns, no, d, ls, lg);
-            s = tr.x;
-            c.rkx[j + 1] = lg;
-            c.rks[j + 1] = s;
-        }
+        const Quad q;
+        double v = q.load(c.rk + (long)j * S2, c.sz[s], no);
+        double d;
+        const int j0 = j;
+        quad_walk<false>(c, q, v, s, lg, no, j0 > 0 ? c.wtau[j0 - 1] : ta, c.wtau + j0, c.wcode + j0, to - j0,
+                         c.wtau[to - 1], d, w, [&](int k, double vk, int sk, int nsk, double lgk) {
+                             q.store(c.rk + (long)(j0 + k + 1) * S2, nsk, no, vk);
+                             c.rkx[j0 + k + 1] = lgk;
+                             c.rks[j0 + k + 1] = sk;
+                         });
         w.commit(c);
     }
     c.h.rk_n = max(c.h.rk_n, to + 1);
@@ -1033,7 +1242,6 @@ DEV void lb_extend(Ctx &c, int nw, int to)
 {
     constexpr int S2 = NM * NM;
     const int no = c.sz[c.h.outer];
-    const LaneSlots<NM> ls(no);
     const int top = (c.h.nwin > 1 ? c.h.wpos : 0) + 2;
     const double tb = bound_tau(top, c.h.nwin, c.M.beta);
     int m = c.h.lb_n - 1;
@@ -1047,18 +1255,18 @@ DEV void lb_extend(Ctx &c, int nw, int to)
         Work w;
         int s = c.lbs[m];
         double lg = c.lbx[m];
-        for (; m < to; ++m) {
-            const int j = nw - 1 - m;
-            const double tu = m > 0 ? c.wtau[j + 1] : tb, tj = c.wtau[j];
-            const int4 tr = c.trans(s, c.wcode[j], true);
-            const int ns = c.sz[s];
-            const double d = prop_e(c, c.ev + c.off[s], ns, tu - tj, lg);
-            w.op(tr.y, ns, no);
-            apply_small<NM>(c.opsp + tr.z, c.lb + (long)m * S2, c.lb + (long)(m + 1) * S2, tr.y, ns, no, d, ls, lg);
-            s = tr.x;
-            c.lbx[m + 1] = lg;
-            c.lbs[m + 1] = s;
-        }
+        const Quad q;
+        double v = q.load(c.lb + (long)m * S2, c.sz[s], no);
+        double d;
+        const int m0 = m;
+        // the window operators nw-to .. nw-1-m0, walked downward
+        quad_walk<true>(c, q, v, s, lg, no, m0 > 0 ? c.wtau[nw - m0] : tb, c.wtau + (nw - to),
+                        c.wcode + (nw - to), to - m0, c.wtau[nw - to], d, w,
+                        [&](int k, double vk, int sk, int nsk, double lgk) {
+                            q.store(c.lb + (long)(m0 + k + 1) * S2, nsk, no, vk);
+                            c.lbx[m0 + k + 1] = lgk;
+                            c.lbs[m0 + k + 1] = sk;
+                        });
         w.commit(c);
     }
     c.h.lb_n = max(c.h.lb_n, to + 1);
@@ -2355,6 +2563,10 @@ DEV Ctx make_ctx(const DevModel &M, const ChainLayout &L, char *base, int chain,
         c.lbx = reinterpret_cast<double *>(lds + L.l_lbx);
         c.rks = reinterpret_cast<int *>(lds + L.l_rks);
         c.lbs = reinterpret_cast<int *>(lds + L.l_lbs);
+        c.qa = reinterpret_cast<double *>(lds + L.l_qa);
+        c.qlc = reinterpret_cast<double *>(lds + L.l_qlc);
+        c.qrec = reinterpret_cast<int4 *>(lds + L.l_qrec);
+        c.qsst = reinterpret_cast<int *>(lds + L.l_qsst);
     }
     {
         double *lc = reinterpret_cast<double *>(lds + L.l_legc);

@@ -68,7 +68,7 @@ struct ChainLayout {
     int lcap_lds;         // operators each LDS list holds (window list, trial list)
     // prefix kets / suffix bras of the window's operators (blocks up to 4 states)
     int sc;               // window operators they cover (0: not kept)
-    long l_rk, l_lb, l_rkx, l_lbx, l_rks, l_lbs;
+    long l_rk, l_lb, l_rkx, l_lbx, l_rks, l_lbs, l_qa, l_qlc, l_qrec, l_qsst;
 };
 
 }  // namespace w2d
