@@ -277,6 +277,8 @@ struct Ctx {
     int *qsst;
     const double *legc;  // Legendre recursion coefficients (LDS)
     int nm;              // padded block size used for matrix strides
+    // algorithmic work of this launch, added to the header by save_ctx
+    double wflops = 0.0, wbytes = 0.0;
     DEV Ctx(const DevModel &m, const ChainLayout &l, char *b, ChainHdr &hd) : M(m), L(l), base(b), h(hd) {}
     DEV double *mi(int f) { return minv + (long)f * L.kcap * L.kcap; }
     DEV int fl(int c) const { return op_orb(c) + M.nb * op_spin(c); }
@@ -417,8 +419,8 @@ struct Work {
     }
     DEV void commit(Ctx &c) const
     {
-        c.h.alg_flops += (double)nout * (2.0 * (double)ntns + (double)nss);
-        c.h.alg_bytes += 8.0 * (double)ntns + 8.0 * nout * (double)(nss + nts);
+        c.wflops += (double)(nout * (2 * ntns + nss));
+        c.wbytes += (double)(8 * ntns + 8LL * nout * (nss + nts));
     }
 };
 
@@ -650,16 +652,45 @@ DEV double quad_apply(double v, const double (&p)[4], double &lg)
     return w;
 }
 
-// final propagation (lane factor d) and normalisation to a largest modulus of one
+// final propagation (lane factor d); renormalised by a power of two only when
+// the largest entry leaves [2^-64, 2^64]
 DEV double quad_finish(double v, double d, double &lg)
 {
     v *= d;
-    const double m = wave_max(fabs(v));
-    if (m > 0.0) {
-        v *= 1.0 / m;
-        lg += log(m);
+    const double m = fabs(v);
+    if (__ballot(m >= STATE_TINY) == 0ull || __ballot(m > STATE_HUGE) != 0ull) {
+        const double mm = wave_max(m);
+        if (mm > 0.0) {
+            const int e = ilogb(mm);
+            lg += (double)e * 0.69314718055994530942;
+            v = ldexp(v, -e);
+        }
     }
     return v;
+}
+
+// Trace of a quad ket v (superstate s, log scale lg) against the row-major
+// bra B at the same time: column sums by row rotations, part traces of the
+// outer states to c.pl, the total in (tl, ts) (the plain sum of the columns:
+// every column carries the same scale exp(lg)).
+DEV void quad_contract(Ctx &c, const Quad &q, double v, const double *B, int s, int no, double lg, double &tl,
+                       double &ts)
+{
+    const double x = v * q.load(B, c.sz[s], no);
+    const double col = (x + dpp_d<DPP_ROW_ROR8>(x)) + (dpp_d<DPP_ROW_ROR4>(x) + dpp_d<DPP_ROW_ROR12>(x));
+    if (lane_id() < no) {
+        c.pl[2 * lane_id()] = lg + log(fabs(col));
+        c.pl[2 * lane_id() + 1] = copysign(1.0, col);
+    }
+    const double S = (readlane_d(col, 0) + readlane_d(col, 1)) + (readlane_d(col, 2) + readlane_d(col, 3));
+    if (S == 0.0) {
+        tl = -DBL_MAX;
+        ts = 0.0;
+    } else {
+        tl = lg + log(fabs(S));
+        ts = copysign(1.0, S);
+    }
+    wsync();
 }
 
 constexpr int QCH = 16;   // operators per chunk of a quad walk
@@ -684,13 +715,20 @@ DEV bool quad_walk(Ctx &c, const Quad &q, double &v, int &s, double &lg, int no,
     // pass (2): lane l computes entry (r, m) = (q.r, q.i), i.e. A(r, k[m])
     const int km = q.i == 0 ? q.k[0] : q.i == 1 ? q.k[1] : q.i == 2 ? q.k[2] : q.k[3];
     int ns = c.sz[s], eo = c.off[s];
+    const int nb4 = 4 * c.M.nb;
     double tlast = tstart;
     for (int c0 = 0; c0 < n; c0 += QCH) {
         const int nc = min(QCH, n - c0);
-        // (1) path: record {block offset, target size, source size, source energies offset}
+        // (1) path: record {block offset, target size, source size, source
+        // energies offset}; lane kk decodes the column of operator kk first
+        int colv = 0;
+        if (l < nc) {
+            const int cd = ocode[BRA ? n - 1 - (c0 + l) : c0 + l];
+            const int ca = BRA ? 2 - op_ca(cd) : op_ca(cd) - 1;
+            colv = (op_orb(cd) * 2 + op_spin(cd)) * 2 + ca;
+        }
         for (int kk = 0; kk < nc; ++kk) {
-            const int idx = BRA ? n - 1 - (c0 + kk) : c0 + kk;
-            const int4 rec = c.trans(s, ocode[idx], BRA);
+            const int4 rec = c.tr[s * nb4 + __builtin_amdgcn_readlane(colv, kk)];
             if (rec.x < 0) return false;
             c.qrec[kk] = make_int4(rec.z, rec.y, ns, eo);
             c.qsst[kk] = rec.x;
@@ -723,17 +761,25 @@ DEV bool quad_walk(Ctx &c, const Quad &q, double &v, int &s, double &lg, int no,
             }
         }
         wsync();
-        // (3) products
+        // (3) products; the next operator's entries, log factor and target
+        // are read one operator ahead
         const double *pa = c.qa + q.r * 4;
         double p[4] = {pa[0], pa[1], pa[2], pa[3]};
+        double lcv = c.qlc[0];
+        int sv = c.qsst[0], ntv = c.qrec[0].y;
         for (int kk = 0; kk < nc; ++kk) {
             const int kn = kk + 1 < nc ? kk + 1 : kk;
             const double pn[4] = {pa[kn * 16], pa[kn * 16 + 1], pa[kn * 16 + 2], pa[kn * 16 + 3]};
-            lg += c.qlc[kk];
+            const double lcn = c.qlc[kn];
+            const int svn = c.qsst[kn], ntn = c.qrec[kn].y;
+            lg += lcv;
             v = quad_apply(v, p, lg);
-            after(c0 + kk, v, c.qsst[kk], c.qrec[kk].y, lg);
+            after(c0 + kk, v, sv, ntv, lg);
 #pragma unroll
             for (int m = 0; m < 4; ++m) p[m] = pn[m];
+            lcv = lcn;
+            sv = svn;
+            ntv = ntn;
         }
         tlast = otau[BRA ? n - c0 - nc : c0 + nc - 1];
         wsync();
@@ -1182,10 +1228,21 @@ DEV bool window_trace(Ctx &c, int nl, double &tl, double &ts)
     const int p = c.h.nwin > 1 ? c.h.wpos : 0;
     const int top = p + 2;
     const double ta = bound_tau(p, c.h.nwin, c.M.beta), tb = bound_tau(top, c.h.nwin, c.M.beta);
-    double *K = c.K0, *Kw = c.K1;
     int s = c.h.kst_sst;
-    copy_mat(K, c.kst, c.sz[s] * no);
     double lg = c.h.kst_log;
+    if constexpr (NM <= 4) {
+        const Quad q;
+        double v = q.load(c.kst, c.sz[s], no);
+        double d;
+        Work w;
+        const bool ok = quad_walk<false>(c, q, v, s, lg, no, ta, c.ttau, c.tcode, nl, tb, d, w, NoAfter());
+        w.commit(c);
+        if (!ok || s != c.bndsst[top]) return false;
+        quad_contract(c, q, v * d, c.bnd + (long)top * c.nm * c.nm, s, no, lg + c.bndlog[top], tl, ts);
+        return true;
+    }
+    double *K = c.K0, *Kw = c.K1;
+    copy_mat(K, c.kst, c.sz[s] * no);
     if (!ket_walk<NM>(c, K, Kw, s, lg, no, ta, c.ttau, c.tcode, nl, tb)) return false;
     if (s != c.bndsst[top]) return false;
     contract(c, K, c.bnd + (long)top * c.nm * c.nm, s, no, lg + c.bndlog[top], tl, ts);
@@ -1285,6 +1342,17 @@ DEV bool segment_trace(Ctx &c, int nw, int j0, int m0, int a, int n, double t0, 
     if (c.h.lb_n <= m0) lb_extend<NM>(c, nw, m0);
     int s = c.rks[j0];
     double lg = c.rkx[j0];
+    if constexpr (NM <= 4) {
+        const Quad q;
+        double v = q.load(c.rk + (long)j0 * S2, c.sz[s], no);
+        double d;
+        Work w;
+        const bool ok = quad_walk<false>(c, q, v, s, lg, no, t0, c.ttau + a, c.tcode + a, n, t1, d, w, NoAfter());
+        w.commit(c);
+        if (!ok || s != c.lbs[m0]) return false;
+        quad_contract(c, q, v * d, c.lb + (long)m0 * S2, s, no, lg + c.lbx[m0], tl, ts);
+        return true;
+    }
     double *K = c.K0, *Kw = c.K1;
     copy_mat(K, c.rk + (long)j0 * S2, c.sz[s] * no);
     if (!ket_walk<NM>(c, K, Kw, s, lg, no, t0, c.ttau + a, c.tcode + a, n, t1)) return false;
@@ -1715,8 +1783,8 @@ DEV int step_add(Ctx &c)
     const int fp_c = wave_isum(fpc) + 1, fp_a = wave_isum(fpa) + 1;
     wsync();
     const double S = detrat_add(c, c.mi(f), k, hybr(M, ta - tc, f));
-    c.h.alg_flops += 2.0 * k * k;
-    c.h.alg_bytes += 8.0 * k * k;
+    c.wflops += 2.0 * k * k;
+    c.wbytes += 8.0 * k * k;
     const double sig = (abs(fp_c - fp_a) % 2) ? -1.0 : 1.0;
     const double dr_log = log(fabs(S)), dr_sign = copysign(1.0, S) * sig;
     const double rnd = grnd(c.rng);
@@ -1728,8 +1796,8 @@ DEV int step_add(Ctx &c)
     if (!(rnd < prob)) return OUT_REJ;
     // accept: M, operator list, flavour lists, weights, sign
     mat_add_phys(c, c.mi(f), k, S);
-    c.h.alg_flops += 6.0 * k * k;
-    c.h.alg_bytes += 8.0 * k * k + 8.0 * (k + 1) * (k + 1);
+    c.wflops += 6.0 * k * k;
+    c.wbytes += 8.0 * k * k + 8.0 * (k + 1) * (k + 1);
     commit_window(c, nl);
     c.h.pc_valid = 0;
     // segment caches: the first q1 and the last nw - q2 window operators are unchanged
@@ -1868,8 +1936,8 @@ DEV int step_rem(Ctx &c)
         return OUT_REJ;
     }
     mat_rem_phys(c, c.mi(f), k, sc, sa);
-    c.h.alg_flops += 3.0 * k * k;
-    c.h.alg_bytes += 8.0 * k * k + 8.0 * (k - 1) * (k - 1);
+    c.wflops += 3.0 * k * k;
+    c.wbytes += 8.0 * k * k + 8.0 * (k - 1) * (k - 1);
     // the operators that owned the last physical row / column take the freed slots
     int *fcsf = c.fcs + fo, *fasf = c.fas + fo;
     const int rl = find_slot(fcsf, k, k - 1), al = find_slot(fasf, k, k - 1);
@@ -2075,8 +2143,8 @@ DEV int step_glob(Ctx &c)
             dns *= s;
         }
         const double k3 = (double)k * k * k;
-        c.h.alg_flops += (2.0 / 3.0) * k3 * 2.0;
-        c.h.alg_bytes += 8.0 * (double)k * k;
+        c.wflops += (2.0 / 3.0) * k3 * 2.0;
+        c.wbytes += 8.0 * (double)k * k;
     }
     // det(old M) = 1 / det(old Delta), which the chain carries
     dol = -c.h.det_log;
@@ -2630,6 +2698,8 @@ DEV Ctx make_ctx(const DevModel &M, const ChainLayout &L, char *base, int chain,
 
 DEV void save_ctx(Ctx &c)
 {
+    c.h.alg_flops += c.wflops;
+    c.h.alg_bytes += c.wbytes;
     uint32_t *gmt = reinterpret_cast<uint32_t *>(c.base + c.L.mt);
     for (int i = lane_id(); i < 624; i += WAVE) gmt[i] = c.rng.mt[i];
     int *gh = reinterpret_cast<int *>(c.ghdr);
@@ -2682,8 +2752,8 @@ __global__ void __launch_bounds__(WAVE) sweep_kernel(DevModel M, ChainLayout L, 
             ga.kf = c.h.kf, ga.fc = c.fc, ga.fa = c.fa, ga.minv = c.minv, ga.fcs = c.fcs, ga.fas = c.fas;
             ga.gtau = gtau, ga.gleg = gleg;
             const double2 wk = meas_gtau(ga);
-            c.h.alg_flops += wk.x;
-            c.h.alg_bytes += wk.y;
+            c.wflops += wk.x;
+            c.wbytes += wk.y;
         }
     }
     save_ctx(c);
