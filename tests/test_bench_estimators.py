@@ -1,0 +1,48 @@
+"""Statistics of bench.py's G(tau) efficiency figures (host only)."""
+
+import importlib.util
+import os
+
+import numpy as np
+
+from conftest import ROOT
+
+
+def _bench():
+    spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(ROOT, "bench.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def _acc(rng, nchains, signs):
+    """Raw per-chain sums as the engine returns them: 40 measurements per chain,
+    G samples of mean -0.3 with unit noise, signs as given."""
+    nmeas, nf, ntau = 40, 2, 8
+    s = signs(rng, (nchains, nmeas))
+    g = (-0.3 + rng.standard_normal((nchains, nmeas, nf, ntau))) * s[:, :, None, None]
+    sc = np.zeros((nchains, 14))
+    sc[:, 0] = s.sum(1)
+    sc[:, 1] = nmeas
+    return {"gtau": g.sum(1), "scalars": sc}
+
+
+def test_block_ratio_estimator_matches_per_chain_without_sign_problem():
+    b = _bench()
+    rng = np.random.default_rng(7)
+    acc = _acc(rng, 1024, lambda r, sh: np.ones(sh))
+    ntau, beta = 8, 2.0
+    per_chain = acc["gtau"] / acc["scalars"][:, 0][:, None, None] * ntau / beta ** 2
+    ref = b.gtau_inv_err2({"gtau": per_chain})
+    blk = b.gtau_inv_err2_chains(acc, ntau, beta, nblocks=64)
+    # same expectation; 64 block samples give a ~18% standard deviation of 1/err^2
+    assert 0.6 < blk / ref < 1.6
+
+
+def test_block_ratio_estimator_finite_with_zero_sign_chains():
+    b = _bench()
+    rng = np.random.default_rng(8)
+    acc = _acc(rng, 256, lambda r, sh: np.where(r.random(sh) < 0.8, 1.0, -1.0))
+    acc["scalars"][:3, 0] = 0.0        # chains whose own sign sum vanished
+    v = b.gtau_inv_err2_chains(acc, 8, 2.0, nblocks=32)
+    assert np.isfinite(v) and v > 0
