@@ -60,25 +60,33 @@ def gtau_inv_err2(res):
     return 1.0 / float(se2.mean())
 
 
+def ratio_inv_err2(graw, sgn, ntau, beta):
+    """1 / <stderr^2(G(tau))> of the sign-ratio estimate G = sum G_raw / sum sign
+    (the normalisation of CTQMC.F90:8663-8771 applied to pooled sums), by the
+    jackknife over the leading axis (blocks of chains or of measurements).
+    Without a sign problem it is the plain standard error of the block means;
+    with one it stays finite where a single block's sign sum vanishes."""
+    g = np.asarray(graw, dtype=float).reshape(len(sgn), -1)
+    s = np.asarray(sgn, dtype=float)
+    n = g.shape[0]
+    if n < 2:
+        return float("nan")
+    jk = (g.sum(0)[None, :] - g) / (s.sum() - s)[:, None] * ntau / beta ** 2
+    se2 = (n - 1) / n * ((jk - jk.mean(0)) ** 2).sum(0)
+    if not (np.all(np.isfinite(se2)) and float(se2.mean()) > 0.0):
+        print("G(tau) jackknife carries no spread or no sign; 1/err^2 undefined", file=sys.stderr)
+        return float("nan")
+    return 1.0 / float(se2.mean())
+
+
 def gtau_inv_err2_chains(acc, ntau, beta, nblocks=64):
-    """1 / <stderr^2(G(tau))> of a batched run over blocks of chains, each block
-    the sign-ratio estimate sum_c G_c / sum_c (sum of signs)_c (the normalisation
-    of CTQMC.F90:8663-8771 applied to the block's pooled sums).  Without a sign
-    problem its expectation equals that of the per-chain estimate; with one it
-    stays finite where a short chain's own sign sum is zero."""
+    """Batched run: the chains pooled into `nblocks` contiguous blocks."""
     g = np.asarray(acc["gtau"], dtype=float)            # [chain, flavour, ntau] raw sums
     sg = np.asarray(acc["scalars"], dtype=float)[:, 0]  # sum of signs per chain
     nb = min(nblocks, g.shape[0])
-    if nb < 2:
-        return float("nan")
     blocks = np.array_split(np.arange(g.shape[0]), nb)
-    gb = np.stack([g[i].sum(0) / sg[i].sum() for i in blocks]) * ntau / beta ** 2
-    m = gb.mean(0)
-    se2 = ((gb - m) ** 2).sum(0) / (nb * (nb - 1))
-    if not float(se2.mean()) > 0.0:
-        print("G(tau) block samples carry no spread; 1/err^2 undefined", file=sys.stderr)
-        return float("nan")
-    return 1.0 / float(se2.mean())
+    return ratio_inv_err2(np.stack([g[i].sum(0) for i in blocks]), np.array([sg[i].sum() for i in blocks]),
+                          ntau, beta)
 
 
 def run_gpu(prob, tdm, nchains, capacity, warm_sweeps, steps, ncorr, seed):
@@ -113,17 +121,20 @@ def run_plain(prob, tdm, capacity, warm_sweeps, steps, ncorr, seed, blocks=8):
     gpu.set_taudiffmax(tdm)
     gpu.warmup(warm_sweeps * ncorr)
     per = max(1, steps // blocks)
-    g = []
+    g, sg = [], []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(blocks):
         gpu.measure(per)
-        g.append(gpu.results()["gtau"][0])
+        acc = gpu.accumulators()
+        g.append(np.asarray(acc["gtau"][0], dtype=float))
+        sg.append(float(acc["scalars"][0, 0]))
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     gpu.close()
     moves = blocks * per * ncorr
-    return dict(wall=wall, moves=moves, inv_err2=gtau_inv_err2({"gtau": np.array(g)}))
+    return dict(wall=wall, moves=moves,
+                inv_err2=ratio_inv_err2(np.array(g), np.array(sg), prob.qmc()["Ntau"], prob.beta))
 
 
 def cpu_baseline(prob, tdm, seed, nwarm, nmeas, ncorr):
@@ -244,7 +255,7 @@ def main():
 
     plain = None
     if not args.no_plain:
-        psteps = max(16, args.steps // 4)
+        psteps = max(32, args.steps // 4)
         p = run_plain(prob, tdm, capacity, args.warmup, psteps, args.ncorr, seed)
         plain = dict(moves_per_s=p["moves"] / p["wall"], gtau_inv_err2_per_s=p["inv_err2"] / p["wall"],
                      sweeps=psteps, wall_s=p["wall"])

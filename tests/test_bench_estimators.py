@@ -39,6 +39,19 @@ def test_block_ratio_estimator_matches_per_chain_without_sign_problem():
     assert 0.6 < blk / ref < 1.6
 
 
+def test_jackknife_equals_block_standard_error_without_sign_problem():
+    """For a linear estimator (all signs +1, equal block counts) the jackknife
+    error is the standard error of the block means exactly."""
+    b = _bench()
+    rng = np.random.default_rng(9)
+    acc = _acc(rng, 64, lambda r, sh: np.ones(sh))
+    ntau, beta = 8, 2.0
+    per_block = acc["gtau"] / acc["scalars"][:, 0][:, None, None] * ntau / beta ** 2
+    ref = b.gtau_inv_err2({"gtau": per_block})
+    jk = b.ratio_inv_err2(acc["gtau"], acc["scalars"][:, 0], ntau, beta)
+    assert np.isclose(jk, ref, rtol=1e-10)
+
+
 def test_block_ratio_estimator_finite_with_zero_sign_chains():
     b = _bench()
     rng = np.random.default_rng(8)

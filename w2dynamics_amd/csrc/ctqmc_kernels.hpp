@@ -2428,11 +2428,11 @@ DEV double row_reduce_scatter16(double (&v)[16])
     return keep + dpp_d<DPP_QP_1032>(give);
 }
 
-// MeasGtauRem (CTQMC.F90:2252-2440): lanes over (annihilator, creator) pairs.
-// G(tau) bins by atomic adds; the Legendre sums run the Bonnet recursion per
-// lane and fold 16 orders at a time with row_reduce_scatter16 into per-lane
-// accumulators (up to 128 orders per pass over the pairs), combined across
-// rows once per flavour.  Kept out of line: inlined into the sweep kernel, the
+// MeasGtauRem (CTQMC.F90:2252-2440): lanes over (annihilator, creator) pairs,
+// up to four pairs per lane.  G(tau) bins by atomic adds; the Legendre sums run
+// the Bonnet recursion per pair and fold 16 orders at a time with
+// row_reduce_scatter16 into per-lane accumulators (up to 128 orders per pass
+// over the pairs), combined across rows once per flavour.  Kept out of line: inlined into the sweep kernel, the
 // ROCm 7.2 backend emits an invalid V_CMP (src_shared_base operand) for it.
 struct GtauArgs {
     int nf, ntau, nleg, kcap;
@@ -2444,16 +2444,89 @@ struct GtauArgs {
     double *gtau, *gleg;
 };
 
+// One pass over the pairs of flavour f for the Legendre orders [l0, l0 + 128):
+// every lane takes PPL pairs (p = p0 + lane + 64u), runs their Bonnet
+// recursions side by side and folds the PPL products of each order before the
+// reduce-scatter, so one reduce-scatter serves 64 PPL pairs.
+template <int PPL>
+DEV void gtau_pass(const GtauArgs &a, int f, int l0, double (&acc)[8])
+{
+    constexpr int LPASS = 128, NCH = LPASS / 16;
+    const int ld = a.kcap;
+    const int k = a.kf[f];
+    const double *ctl = a.fc + (long)f * a.kcap, *atl = a.fa + (long)f * a.kcap;
+    const int *csl = a.fcs + (long)f * a.kcap, *asl = a.fas + (long)f * a.kcap;
+    const double *Mf = a.minv + (long)f * a.kcap * a.kcap;
+    const int npair = k * k;
+    const int lend = min(a.nleg, l0 + LPASS);
+    for (int p0 = 0; p0 < npair; p0 += WAVE * PPL) {
+        double Mv[PPL], x[PPL], pl[PPL], plm1[PPL];
+#pragma unroll
+        for (int u = 0; u < PPL; ++u) {
+            const int p = p0 + lane_id() + WAVE * u;
+            Mv[u] = 0.0;
+            x[u] = 0.0;
+            if (p < npair) {
+                const int ai = p / k, ci = p - ai * k;    // annihilator rank, creator rank
+                const double ta = atl[ai], tc = ctl[ci];
+                double m = Mf[csl[ci] + (long)asl[ai] * ld];
+                int idt;
+                if (ta - tc > 0.0) {
+                    idt = (int)((ta - tc) / a.beta * (double)a.ntau) + 1;
+                    x[u] = 2.0 / a.beta * (ta - tc) - 1.0;
+                } else {
+                    idt = (int)((a.beta + ta - tc) / a.beta * (double)a.ntau) + 1;
+                    x[u] = 2.0 / a.beta * (a.beta + ta - tc) - 1.0;
+                    m = -m;
+                }
+                Mv[u] = m * a.cfgsign;
+                if (idt > a.ntau) idt = a.ntau;
+                if (l0 == 0) global_add(&a.gtau[(long)f * a.ntau + idt - 1], Mv[u]);
+            }
+            pl[u] = 0.0;
+            plm1[u] = 0.0;
+        }
+        // Bonnet recursion from order 0 up to the start of this pass
+        for (int l = 0; l < l0; ++l) {
+#pragma unroll
+            for (int u = 0; u < PPL; ++u) {
+                const double plp1 = l == 0 ? 1.0 : (l == 1 ? x[u] : a.legc[2 * (l - 1)] * x[u] * pl[u] - a.legc[2 * (l - 1) + 1] * plm1[u]);
+                plm1[u] = pl[u];
+                pl[u] = plp1;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < NCH; ++q) {
+            const int lb = l0 + 16 * q;
+            if (lb >= lend) break;
+            double v[16];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const int l = lb + j;
+                double sum = 0.0;
+                if (l < lend) {
+                    const double c1 = l >= 2 ? a.legc[2 * (l - 1)] : 0.0, c2 = l >= 2 ? a.legc[2 * (l - 1) + 1] : 0.0;
+#pragma unroll
+                    for (int u = 0; u < PPL; ++u) {
+                        const double plp1 = l == 0 ? 1.0 : (l == 1 ? x[u] : c1 * x[u] * pl[u] - c2 * plm1[u]);
+                        plm1[u] = pl[u];
+                        pl[u] = plp1;
+                        sum = fma(Mv[u], plp1, sum);
+                    }
+                }
+                v[j] = sum;
+            }
+            acc[q] += row_reduce_scatter16(v);
+        }
+    }
+}
+
 __device__ __attribute__((noinline)) double2 meas_gtau(const GtauArgs a)
 {
-    const int ld = a.kcap;
     constexpr int LPASS = 128, NCH = LPASS / 16;
     double wflops = 0.0, wbytes = 0.0;
     for (int f = 0; f < a.nf; ++f) {
         const int k = a.kf[f];
-        const double *ctl = a.fc + (long)f * a.kcap, *atl = a.fa + (long)f * a.kcap;
-        const int *csl = a.fcs + (long)f * a.kcap, *asl = a.fas + (long)f * a.kcap;
-        const double *Mf = a.minv + (long)f * a.kcap * a.kcap;
         wflops += (double)k * k * (5.0 * a.nleg + 4.0);
         wbytes += 8.0 * k * k;
         const int npair = k * k;
@@ -2461,53 +2534,9 @@ __device__ __attribute__((noinline)) double2 meas_gtau(const GtauArgs a)
             double acc[NCH];
 #pragma unroll
             for (int q = 0; q < NCH; ++q) acc[q] = 0.0;
-            for (int p0 = 0; p0 < npair; p0 += WAVE) {
-                const int p = p0 + lane_id();
-                double Mv = 0.0, x = 0.0;
-                if (p < npair) {
-                    const int ai = p / k, ci = p - ai * k;    // annihilator rank, creator rank
-                    const double ta = atl[ai], tc = ctl[ci];
-                    Mv = Mf[csl[ci] + (long)asl[ai] * ld];
-                    int idt;
-                    if (ta - tc > 0.0) {
-                        idt = (int)((ta - tc) / a.beta * (double)a.ntau) + 1;
-                        x = 2.0 / a.beta * (ta - tc) - 1.0;
-                    } else {
-                        idt = (int)((a.beta + ta - tc) / a.beta * (double)a.ntau) + 1;
-                        x = 2.0 / a.beta * (a.beta + ta - tc) - 1.0;
-                        Mv = -Mv;
-                    }
-                    Mv = Mv * a.cfgsign;
-                    if (idt > a.ntau) idt = a.ntau;
-                    if (l0 == 0) global_add(&a.gtau[(long)f * a.ntau + idt - 1], Mv);
-                }
-                // Bonnet recursion from order 0 up to the end of this pass
-                double pl = 0.0, plm1 = 0.0;
-                const int lend = min(a.nleg, l0 + LPASS);
-                for (int l = 0; l < l0; ++l) {
-                    const double plp1 = l == 0 ? 1.0 : (l == 1 ? x : a.legc[2 * (l - 1)] * x * pl - a.legc[2 * (l - 1) + 1] * plm1);
-                    plm1 = pl;
-                    pl = plp1;
-                }
-#pragma unroll
-                for (int q = 0; q < NCH; ++q) {
-                    const int lb = l0 + 16 * q;
-                    if (lb >= lend) break;
-                    double v[16];
-#pragma unroll
-                    for (int j = 0; j < 16; ++j) {
-                        const int l = lb + j;
-                        double plp1 = 0.0;
-                        if (l < lend) {
-                            plp1 = l == 0 ? 1.0 : (l == 1 ? x : a.legc[2 * (l - 1)] * x * pl - a.legc[2 * (l - 1) + 1] * plm1);
-                            plm1 = pl;
-                            pl = plp1;
-                        }
-                        v[j] = Mv * plp1;
-                    }
-                    acc[q] += row_reduce_scatter16(v);
-                }
-            }
+            if (npair > 2 * WAVE) gtau_pass<4>(a, f, l0, acc);
+            else if (npair > WAVE) gtau_pass<2>(a, f, l0, acc);
+            else gtau_pass<1>(a, f, l0, acc);
             // rows -> lanes 0..15, added to the chain's Legendre sums
 #pragma unroll
             for (int q = 0; q < NCH; ++q) {
