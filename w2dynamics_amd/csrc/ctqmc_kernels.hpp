@@ -2521,7 +2521,9 @@ DEV void gtau_pass(const GtauArgs &a, int f, int l0, double (&acc)[8])
     }
 }
 
-__device__ __attribute__((noinline)) double2 meas_gtau(const GtauArgs a)
+// all flavours; WIDE: flavours of more than 64 pairs take 2 or 4 pairs per lane
+template <bool WIDE>
+DEV double2 gtau_all(const GtauArgs &a)
 {
     constexpr int LPASS = 128, NCH = LPASS / 16;
     double wflops = 0.0, wbytes = 0.0;
@@ -2534,8 +2536,8 @@ __device__ __attribute__((noinline)) double2 meas_gtau(const GtauArgs a)
             double acc[NCH];
 #pragma unroll
             for (int q = 0; q < NCH; ++q) acc[q] = 0.0;
-            if (npair > 2 * WAVE) gtau_pass<4>(a, f, l0, acc);
-            else if (npair > WAVE) gtau_pass<2>(a, f, l0, acc);
+            if (WIDE && npair > 2 * WAVE) gtau_pass<4>(a, f, l0, acc);
+            else if (WIDE && npair > WAVE) gtau_pass<2>(a, f, l0, acc);
             else gtau_pass<1>(a, f, l0, acc);
             // rows -> lanes 0..15, added to the chain's Legendre sums
 #pragma unroll
@@ -2553,6 +2555,12 @@ __device__ __attribute__((noinline)) double2 meas_gtau(const GtauArgs a)
     wsync();
     return make_double2(wflops, wbytes);
 }
+
+// Two out-of-line entry points: the caller's registers saved around the call
+// depend on the callee's footprint, so kernels whose chains stay at a few
+// pairs per flavour (one-state blocks) call the narrow one.
+__device__ __attribute__((noinline)) double2 meas_gtau(const GtauArgs a) { return gtau_all<true>(a); }
+__device__ __attribute__((noinline)) double2 meas_gtau_narrow(const GtauArgs a) { return gtau_all<false>(a); }
 
 // MeasDensityMatrix_beta_half (CTQMC.F90:3088-3151) with update_b2_states_eb
 // (Trace.F90:6213-6469): ket from 0 to beta/2, bra from beta to beta/2
@@ -2780,7 +2788,7 @@ __global__ void __launch_bounds__(WAVE) sweep_kernel(DevModel M, ChainLayout L, 
             ga.beta = M.beta, ga.cfgsign = c.h.cfgsign, ga.legc = c.legc;
             ga.kf = c.h.kf, ga.fc = c.fc, ga.fa = c.fa, ga.minv = c.minv, ga.fcs = c.fcs, ga.fas = c.fas;
             ga.gtau = gtau, ga.gleg = gleg;
-            const double2 wk = meas_gtau(ga);
+            const double2 wk = NM == 1 ? meas_gtau_narrow(ga) : meas_gtau(ga);
             c.wflops += wk.x;
             c.wbytes += wk.y;
         }
