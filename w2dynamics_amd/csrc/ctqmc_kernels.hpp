@@ -1042,17 +1042,18 @@ template <int NM>
 DEV bool ket_walk_global(Ctx &c, double *&K, double *&Kw, int &s, double &lg, int no, double t0, int j0,
                          int j1, double t1)
 {
+    // one walk call site: an empty range is one chunk of no operators
     double t = t0;
-    for (int b = j0; b < j1;) {
-        int n = stage_chunk(c, b, j1);
-        double tend = (b + n < j1) ? c.ttau[n - 1] : t1;
+    int b = j0;
+    do {
+        const int n = b < j1 ? stage_chunk(c, b, j1) : 0;
+        const double tend = (b + n < j1) ? c.ttau[n - 1] : t1;
         // walk the chunk; the final propagation of a non-last chunk is zero
         if (!ket_walk<NM>(c, K, Kw, s, lg, no, t, c.ttau, c.tcode, n, tend)) return false;
         t = tend;
         b += n;
         wsync();
-    }
-    if (j0 >= j1) ket_walk<NM>(c, K, Kw, s, lg, no, t0, c.ttau, c.tcode, 0, t1);
+    } while (b < j1);
     return true;
 }
 
@@ -1061,16 +1062,16 @@ DEV void bra_walk_global(Ctx &c, double *&B, double *&Bw, int &s, double &lg, in
                          int j1, double t0)
 {
     double t = t1;
-    for (int e = j1; e > j0;) {
-        int b = max(j0, e - c.L.lcap_lds);
-        int n = stage_chunk(c, b, e);
-        double tend = (b > j0) ? c.ttau[0] : t0;
+    int e = j1;
+    do {
+        const int b = max(j0, e - c.L.lcap_lds);
+        const int n = e > j0 ? stage_chunk(c, b, e) : 0;
+        const double tend = (b > j0) ? c.ttau[0] : t0;
         bra_walk<NM>(c, B, Bw, s, lg, no, t, c.ttau, c.tcode, n, tend);
         t = tend;
         e = b;
         wsync();
-    }
-    if (j0 >= j1) bra_walk<NM>(c, B, Bw, s, lg, no, t1, c.ttau, c.tcode, 0, t0);
+    } while (e > j0);
 }
 
 DEV void set_identity(double *K, int no)
