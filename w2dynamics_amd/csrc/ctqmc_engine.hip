@@ -153,8 +153,10 @@ int layout_and_alloc(w2d_ctqmc *h, int nbound)
     L.l_mt = lds(624 * 4);
     L.l_sz = lds((long)m.nsst * 4);
     L.l_off = lds((long)m.nsst * 4);
-    L.l_cn = lds((long)m.nsst * m.nbands * 4 * 4);
-    L.l_oo = lds((long)m.nsst * m.nbands * 4 * 4);
+    // Connect and operator-block offsets: LDS for blocks up to 16 states, HBM
+    // above (with them a chain of the 5-orbital model fits 20 KB, eight per CU)
+    L.l_cn = nm <= 16 ? lds((long)m.nsst * m.nbands * 4 * 4) : -1;
+    L.l_oo = nm <= 16 ? lds((long)m.nsst * m.nbands * 4 * 4) : -1;
     // transition records and energies: LDS for blocks up to 16 states, HBM
     // above (there the block products dominate and LDS buys chains per CU)
     L.l_tr = nm <= 16 ? lds((long)m.nsst * m.nbands * 4 * 16) : -1;
@@ -439,6 +441,7 @@ int w2d_ctqmc_create(const char *params, int nbands, int nftau, const double *u_
     }
     std::vector<int> first_state(m.nsst);
     for (int s = 0; s < m.nsst; ++s) first_state[s] = m.states[m.offsets[s]];
+    std::vector<int> opoff32(m.opoff.begin(), m.opoff.end());
     std::vector<int4> trans(m.connect.size());
     for (size_t i = 0; i < m.connect.size(); ++i) {
         const int ts = m.connect[i];
@@ -450,7 +453,8 @@ int w2d_ctqmc_create(const char *params, int nbands, int nftau, const double *u_
         (st = upload(h.get(), m.state_sst, &d.state_sst)) || (st = upload(h.get(), m.opoff, &d.opoff)) ||
         (st = upload(h.get(), m.ops, &d.ops)) || (st = upload(h.get(), m.evals, &d.evals)) ||
         (st = upload(h.get(), fdiag, &d.ftau)) || (st = upload(h.get(), dmoff, &d.dmoff)) ||
-        (st = upload(h.get(), legc, &d.legc)) || (st = upload(h.get(), trans, &d.trans)))
+        (st = upload(h.get(), legc, &d.legc)) || (st = upload(h.get(), trans, &d.trans)) ||
+        (st = upload(h.get(), opoff32, &d.opoff32)))
         return st;
     d.p_shift = P.real("PercentageTauShiftMove");
     d.p_glob = P.real("PercentageGlobalMove");

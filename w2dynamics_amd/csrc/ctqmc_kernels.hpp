@@ -2699,9 +2699,11 @@ DEV Ctx make_ctx(const DevModel &M, const ChainLayout &L, char *base, int chain,
         off[i] = M.offsets[i];
     }
     const int ncn = M.nsst * M.nb * 4;
-    for (int i = lane_id(); i < ncn; i += WAVE) {
-        cn[i] = M.connect[i];
-        oo[i] = (int)M.opoff[i];
+    if constexpr (NM <= 16) {
+        for (int i = lane_id(); i < ncn; i += WAVE) {
+            cn[i] = M.connect[i];
+            oo[i] = (int)M.opoff[i];
+        }
     }
     if constexpr (NM <= 16) {
         for (int i = lane_id(); i < M.nstates; i += WAVE) ev[i] = M.evals[i];
@@ -2724,7 +2726,9 @@ DEV Ctx make_ctx(const DevModel &M, const ChainLayout &L, char *base, int chain,
     } else {
         c.opsp = M.ops;
     }
-    c.sz = sz, c.off = off, c.cn = cn, c.oo = oo;
+    c.sz = sz, c.off = off;
+    if constexpr (NM <= 16) c.cn = cn, c.oo = oo;
+    else c.cn = M.connect, c.oo = M.opoff32;
     if constexpr (NM <= 16) c.ev = ev;
     else c.ev = M.evals;
     c.rng.mt = mt;

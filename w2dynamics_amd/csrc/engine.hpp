@@ -15,6 +15,7 @@ struct DevModel {
     double beta, egs;
     const int *sizes, *offsets, *connect, *first_state, *state_sst;
     const long *opoff;
+    const int *opoff32;                 // the same offsets as int (HBM operator-block table for blocks above 16 states)
     const double *ops, *evals, *ftau;   // ftau[f][nftau]
     const long *dmoff;                  // block-diagonal density-matrix offsets
     const double *legc;                 // Bonnet coefficients (2l+1)/(l+1), l/(l+1) per l
