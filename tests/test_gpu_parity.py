@@ -38,6 +38,8 @@ CASES = {
     "c2_win": (lambda: P.kanamori_2orb(beta=20.0, TaudiffMax=4.0), 12345, 6000),
     "c3_win": (lambda: P.kanamori_3orb(beta=20.0, TaudiffMax=5.0), 777, 3000),
     "c4_w1": (lambda: P.coulomb_5orb(beta=2.0), 99, 600),
+    # 5-orbital blocks (FP64 MFMA walks, HBM model tables) with a sliding window
+    "c4_win": (lambda: P.coulomb_5orb(beta=4.0, TaudiffMax=1.0), 2024, 800),
 }
 
 
