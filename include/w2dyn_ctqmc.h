@@ -31,6 +31,8 @@ extern "C" {
 #define W2D_ERR_SINGULAR 4     /* "DetRat too small in removal" (CTQMC.F90:4062) */
 #define W2D_ERR_ZERO_TRACE 5   /* "Zero trace initial state" (Trace.F90:668) */
 #define W2D_ERR_STATE 6        /* call out of order (e.g. results before measure) */
+#define W2D_ERR_PARTIAL 7      /* some chains stopped on an error, the others completed:
+                                  their results are valid (w2d_ctqmc_chain_status) */
 
 typedef struct w2d_ctqmc w2d_ctqmc;
 
@@ -52,7 +54,10 @@ typedef struct {
  *   u_matrix   : (2N,2N,2N,2N), flavour index 2*(b-1)+s   (CTQMC.F90:7899)
  *   ftau_full  : (N,2,N,2,nftau), only the flavour-diagonal part is used
  *   muimp_full : (N,2,N,2)
- *   capacity   : maximum pairs per flavour a chain may reach (0 = default 256)
+ *   capacity   : initial pairs per flavour per chain (0 = default 256); doubled
+ *                on demand up to 4096 (w2d_ctqmc_chain_status); a negative
+ *                value fixes the capacity at -capacity (chains that outgrow
+ *                it stop with W2D_ERR_CAPACITY, the others carry on)
  * Screening (Uw) is not supported; offdiag, segment, statesampling, worm and
  * four-operator moves must be off. */
 int w2d_ctqmc_create(const char *params, int nbands, int nftau, const double *u_matrix,
@@ -62,7 +67,8 @@ int w2d_ctqmc_create(const char *params, int nbands, int nftau, const double *u_
 /* Replaces set_taudiffmax (CTQMC.F90:8180). */
 int w2d_ctqmc_set_taudiffmax(w2d_ctqmc *h, double taudiff_max);
 
-/* Replaces ctqmc_warmup (CTQMC.F90:8188): nwarmups Metropolis steps per chain. */
+/* Replaces ctqmc_warmup (CTQMC.F90:8188): nwarmups Metropolis steps per chain.
+ * Returns W2D_ERR_PARTIAL if some (not all) chains stopped on an error. */
 int w2d_ctqmc_warmup(w2d_ctqmc *h, int64_t nwarmups);
 
 /* Replaces ctqmc_measure(1,1) (CTQMC.F90:8355): nmeas x (NCorr steps + Z-space
@@ -89,6 +95,18 @@ int w2d_ctqmc_dims(const w2d_ctqmc *h, int *nbands, int *nchains, int *ntau, int
  * flavour index f = b + N*s (orbital fastest). */
 int w2d_ctqmc_get_accumulators(w2d_ctqmc *h, double *gtau, double *gleg, double *dm,
                                double *histo, double *outer, double *scalars);
+
+/* The same accumulators summed over `ngroups` contiguous groups of chains
+ * (group g = chains [g*nchains/ngroups, (g+1)*nchains/ngroups)), leaving out
+ * chains stopped on an error; the sum runs on the GPU.  Shapes as above with
+ * the group in place of the chain, except
+ *   dm_blocks [ngroups][sum_s n_s^2]  the density matrix's diagonal blocks in
+ *             superstate order, each n_s x n_s column-major
+ *   nused     [ngroups]  chains summed into each group
+ * Pooled sums give the sign-ratio estimate sum(G sign) / sum(sign) of the
+ * normalisation CTQMC.F90:8663-8771 and, over groups, its jackknife error. */
+int w2d_ctqmc_get_group_sums(w2d_ctqmc *h, int ngroups, double *gtau, double *gleg, double *dm_blocks,
+                             double *histo, double *outer, double *scalars, int32_t *nused);
 
 /* Eigenbasis of the local Hamiltonian built by create: per-superstate sizes and
  * offsets, eigenvalues (EigenstatesEnergies), occupation-basis state of each
@@ -121,6 +139,34 @@ int w2d_ctqmc_get_minv(w2d_ctqmc *h, int chain, int f, int *k, double *out);
 /* Kernel time of the last warmup/measure call in milliseconds (HIP events on
  * the handle's stream around its launches) and the number of launches. */
 int w2d_ctqmc_last_kernel_ms(const w2d_ctqmc *h, double *ms, int64_t *launches);
+
+/* Per-chain outcome of the last warmup / measure: status[c] = W2D_OK or the
+ * error that stopped chain c (W2D_ERR_SINGULAR for the reference's "DetRat too
+ * small in removal" stop, CTQMC.F90:4062, W2D_ERR_CAPACITY for a chain that
+ * would outgrow 4096 pairs per flavour); a stopped chain's accumulators are
+ * left out by the hosts and the other chains carry on.  kfmax[c][2N] = the
+ * largest order each flavour of chain c has reached; *capacity = current
+ * pairs-per-flavour capacity.  A chain that runs out of room is not an error:
+ * it stops before the move's first random draw, the capacity of every chain is
+ * doubled and it continues (the reference allocates on demand,
+ * MatrixUpdate.F90:410-482).  Any pointer may be NULL.  No reference
+ * counterpart (the reference stops the process, one rank per chain). */
+int w2d_ctqmc_chain_status(const w2d_ctqmc *h, int32_t *status, int32_t *kfmax, int *capacity);
+
+/* Abort word of the handle: flag != 0 makes running and later sweeps stop at
+ * the next measurement boundary (every 128 moves in warm-up), keeping the data
+ * measured so far, as the reference does when a trapped signal fires
+ * (any_signal_fired, CTQMC.F90:8212, 8436, 8648-8658); flag = 0 clears it.
+ * Async-signal-safe (one store to pinned host memory). */
+int w2d_ctqmc_set_abort(w2d_ctqmc *h, int flag);
+
+/* `aborted` of the last warmup / measure (CTQMC.F90:8648-8658): 0 = ran to the
+ * end, 1 = stopped early with data, 2 = stopped before any measurement. */
+int w2d_ctqmc_aborted(const w2d_ctqmc *h, int *aborted);
+
+/* register_trap for SIGTERM, SIGINT and SIGUSR1 (CTQMC.F90:8076-8078): the
+ * handler raises the abort word of every live handle. */
+int w2d_ctqmc_trap_signals(void);
 
 /* Replaces dest_ctqmc (CTQMC.F90:901). */
 int w2d_ctqmc_destroy(w2d_ctqmc *h);

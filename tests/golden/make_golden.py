@@ -162,6 +162,57 @@ def case_run(name: str, prob: P.Problem, seed: int, nwarm: int):
     print(name, "sign", res["mean_sign"], "n_up", res["occ"][0, 0, 0, 0])
 
 
+def case_log_long(name: str, prob: P.Problem, seed: int, nmoves: int, every: int = 10):
+    """A long move log: the full decision sequence (kind, outcome, order, outer
+    superstate, window) and signs of every move, the trace and determinant logs
+    of every `every`-th move (keeps the fixture small)."""
+    with tempfile.TemporaryDirectory() as d:
+        write_inputs(d, prob, "log", seed, 0, nmoves)
+        run_driver(d)
+        moves = np.fromfile(os.path.join(d, "moves.bin"), dtype=MOVE_DT)
+    out = problem_arrays(prob)
+    out.update({"seed": seed, "nmoves": nmoves, "every": every})
+    for f in ("kind", "outcome", "window"):
+        out["move_" + f] = moves[f].astype(np.int8)
+    for f in ("noper", "outer_sst"):
+        out["move_" + f] = moves[f].astype(np.int16)
+    for f in ("trace_sign", "det_sign", "cfgsign"):
+        out["move_" + f] = moves[f].astype(np.int8)
+    for f in ("trace_log", "det_log"):
+        out["move_" + f] = moves[f][every - 1::every]
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
+    print(name, "moves", len(moves), "max order", int(moves["noper"].max()), "accepted",
+          int((moves["outcome"] == 1).sum()))
+
+
+def case_refstat(name: str, prob: P.Problem, seeds, nwarm: int):
+    """Independent reference runs (one per seed, as MPI ranks NSeed + i): mean
+    and standard error over the runs of G(tau), Legendre G, occupations and sign."""
+    q = prob.qmc()
+    nb, ntau, nleg = prob.nbands, int(q["Ntau"]), int(q["NLegMax"])
+    vals = {"gtau": [], "gleg": [], "occ": [], "mean_sign": []}
+    for seed in seeds:
+        with tempfile.TemporaryDirectory() as d:
+            write_inputs(d, prob, "run", seed, nwarm, 0)
+            run_driver(d)
+            raw = np.fromfile(os.path.join(d, "results.bin"), dtype="<f8")
+        off = 0
+        for key, n, shape in (("mean_sign", 1, ()), ("gtau", nb * 2 * ntau, (nb, 2, ntau)),
+                              ("gleg", nb * 2 * nleg, (nb, 2, nleg)),
+                              ("occ", (2 * nb) ** 2, (nb, 2, nb, 2))):
+            a = raw[off:off + n]
+            off += n
+            vals[key].append(a.reshape(shape, order="F") if shape else a[0])
+    out = problem_arrays(prob)
+    out.update({"seeds": np.array(seeds), "nwarm": nwarm, "nmeas": int(q["Nmeas"])})
+    for k, v in vals.items():
+        v = np.array(v)
+        out["mean_" + k] = v.mean(0)
+        out["err_" + k] = v.std(0, ddof=1) / np.sqrt(len(v))
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
+    print(name, "runs", len(seeds), "n_up", out["mean_occ"][0, 0, 0, 0], "+-", out["err_occ"][0, 0, 0, 0])
+
+
 CASES = {
     # single window (TaudiffMax <= 0), 1 orbital, scalar trace path
     "c1_log_w1": lambda: case_log("c1_log_w1", P.anderson_1orb(), 43890, 20000),
@@ -178,6 +229,17 @@ CASES = {
     "c2_run": lambda: case_run("c2_run", P.kanamori_2orb(beta=10.0, Nmeas=500, NCorr=40,
                                                          Ntau=100, NLegMax=20,
                                                          TaudiffMax=3.0), 2024, 5000),
+    # the configured temperatures, long logs (the first 1e5 decisions)
+    "c1_log_1e5": lambda: case_log_long("c1_log_1e5", P.anderson_1orb(TaudiffMax=2.0), 31, 100000),
+    "c2_log_1e5": lambda: case_log_long("c2_log_1e5", P.kanamori_2orb(beta=50.0, TaudiffMax=5.0),
+                                        43890, 100000),
+    "c3_log_b100": lambda: case_log_long("c3_log_b100", P.kanamori_3orb(beta=100.0, TaudiffMax=8.0),
+                                         777, 10000, every=1),
+    "c4_log_b40": lambda: case_log_long("c4_log_b40", P.coulomb_5orb(beta=40.0, TaudiffMax=5.0),
+                                        2048, 1500, every=1),
+    # 16 independent reference runs for the statistical comparison of batched chains
+    "c2_refstat": lambda: case_refstat("c2_refstat", P.kanamori_2orb(
+        beta=10.0, Nmeas=20000, NCorr=40, Ntau=50, NLegMax=20, TaudiffMax=3.0), list(range(500, 516)), 5000),
 }
 
 

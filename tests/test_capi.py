@@ -107,3 +107,47 @@ def test_semicircle_ftau_sum_rule():
     assert abs(f[0] + f[-1] - 1.0) < 1e-10
     assert np.allclose(f, f[::-1], atol=1e-12)
     assert (f > 0).all()
+
+
+def _synthetic_sums(signs, cnt=10.0, nb=1, ntau=4, nleg=3, seed=0):
+    rng = np.random.default_rng(seed)
+    n = len(signs)
+    sc = np.zeros((n, 14))
+    sc[:, 0] = signs
+    sc[:, 1] = cnt
+    sc[:, 2:12] = 1.0
+    return dict(gtau=rng.normal(size=(n, 2 * nb, ntau)), gleg=rng.normal(size=(n, 2 * nb, nleg)),
+                dm=rng.normal(size=(n, 4, 4)), histo=np.ones((n, 2 * nb, 6)),
+                outer_hist=np.ones((n, 4)), scalars=sc)
+
+
+MODEL1 = dict(offsets=np.arange(4), sizes=np.ones(4, int), evecs=[np.eye(1)] * 4,
+              states=np.array([0, 1, 2, 3]))
+
+
+def test_pooled_estimate_is_ratio_of_sums():
+    """The pooled estimate is sum(G) / sum(sign) (CTQMC.F90:8663-8771 on pooled
+    sums), and its jackknife error is the textbook leave-one-out formula."""
+    nb, ntau, nleg, beta = 1, 4, 3, 2.0
+    s = _synthetic_sums([-10.0, -6.0, -8.0, -9.0])
+    r = ctqmc.pooled_estimate(s, nb, beta, ntau, nleg, MODEL1)
+    g = s["gtau"].sum(0).reshape(2, 1, ntau).swapaxes(0, 1) / s["scalars"][:, 0].sum() * ntau / beta ** 2
+    assert np.allclose(r["gtau"], g)
+    assert np.isclose(r["mean_sign"], s["scalars"][:, 0].sum() / 40.0)
+    jk = np.array([(s["gtau"].sum(0) - s["gtau"][i]) / (s["scalars"][:, 0].sum() - s["scalars"][i, 0])
+                   for i in range(4)]) * ntau / beta ** 2
+    err = np.sqrt(3 / 4 * ((jk - jk.mean(0)) ** 2).sum(0))
+    assert np.allclose(r["gtau_err"], err.reshape(2, 1, ntau).swapaxes(0, 1))
+
+
+def test_pooled_estimate_survives_a_zero_sign_chain():
+    """One chain whose sign sum vanishes: its own normalisation is undefined
+    (NaN, as a reference rank would divide by zero) but the pooled estimate and
+    its error stay finite."""
+    nb, ntau, nleg, beta = 1, 4, 3, 2.0
+    s = _synthetic_sums([-10.0, 0.0, -8.0, -9.0])
+    per_chain = ctqmc.normalise(s, nb, beta, ntau, nleg, MODEL1)
+    assert not np.all(np.isfinite(per_chain["gtau"][1]))
+    r = ctqmc.pooled_estimate(s, nb, beta, ntau, nleg, MODEL1)
+    for k in ("gtau", "gleg", "densitymatrix", "occ", "mean_sign"):
+        assert np.all(np.isfinite(r[k])) and np.all(np.isfinite(r[k + "_err"])), k

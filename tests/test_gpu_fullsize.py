@@ -1,0 +1,297 @@
+"""GPU parity at the configured sizes.
+
+* Move logs at the configured temperatures and capacities against the CPU
+  oracle and against the reference's own logs (2-orbital Kanamori at beta=50,
+  3-orbital at beta=100, 5-orbital Coulomb at beta=40), and the first 1e5
+  decisions of a 1- and a 2-orbital chain against the reference.
+* Whole batches at their configured chain counts, thermalised: for sampled
+  chains the inverse hybridisation blocks satisfy M Delta = 1 on the host and
+  the oracle's full recomputation of the configuration (set_config) gives the
+  same trace, determinant and sign.
+* Measurements against the oracle with 9- and 100-state blocks (the
+  density-matrix paths of blocks up to 16 and up to 128 states), 100 and 150
+  Legendre orders and more than 8 pairs per flavour (the wide G(tau) passes).
+* Statistics of batched chains: G(tau), occupations and double occupancy
+  within 3 sigma of exact diagonalisation (discrete bath) and of 16
+  independent reference runs.
+"""
+
+import numpy as np
+import pytest
+
+from conftest import load_golden, qmc_of
+from oracle import OracleChain
+from test_gpu_parity import FIELDS, TOL_LOG, _compare
+from test_oracle import _anderson_ed
+from w2dynamics_amd import problems as P
+from w2dynamics_amd.ctqmc import CtqmcGpu
+
+pytestmark = pytest.mark.gpu
+
+FULL = {
+    # name: (problem, seed, moves, capacity)
+    "c2_b50": (lambda: P.kanamori_2orb(beta=50.0, TaudiffMax=5.0), 4242, 100000, 128),
+    "c3_b100": (lambda: P.kanamori_3orb(beta=100.0, TaudiffMax=8.0), 3131, 10000, 384),
+    "c4_b40": (lambda: P.coulomb_5orb(beta=40.0, TaudiffMax=5.0), 5151, 1500, 64),
+}
+
+
+@pytest.mark.parametrize("case", list(FULL))
+def test_full_temperature_move_log_matches_oracle(case):
+    mk, seed, n, cap = FULL[case]
+    prob = mk()
+    tdm = float(prob.qmc()["TaudiffMax"])
+    print("full-size case", case, "seed", seed, "moves", n, flush=True)
+    gpu = CtqmcGpu.from_problem(prob, nchains=4, seed=seed, capacity=cap)
+    gpu.set_taudiffmax(tdm)
+    rec = gpu.moves_log(0, n)
+    ora = OracleChain.from_problem(prob, seed)
+    ora.set_taudiffmax(tdm)
+    ref = ora.moves(n)
+    _compare(rec, ref, case)
+    assert rec["noper"].max() > 40, "the chain never left small orders"
+    for f in range(2 * prob.nbands):
+        a, b = gpu.get_minv(0, f), ora.minv(f)
+        assert a.shape == b.shape
+        if a.size:
+            assert np.abs(a - b).max() <= 1e-9 * max(1.0, np.abs(b).max())
+    gpu.close()
+
+
+def _golden_problem(z):
+    return CtqmcGpu(str(z["param_string"]), z["umatrix"], z["ftau"], z["muimp"], nchains=1,
+                    seed=int(z["seed"]), capacity=64)
+
+
+@pytest.mark.parametrize("name", ["c1_log_1e5", "c2_log_1e5", "c3_log_b100", "c4_log_b40"])
+def test_long_reference_logs(name):
+    """Every decision of the reference's chain (1e5 moves for 1 and 2 orbitals
+    at beta=10 and 50; the configured beta=100 and beta=40 for 3 and 5
+    orbitals) and its trace and determinant logs (every `every`-th move)."""
+    z = load_golden(name)
+    q = qmc_of(z)
+    print("reference log", name, int(z["nmoves"]), "moves", flush=True)
+    gpu = _golden_problem(z)
+    gpu.set_taudiffmax(float(q["TaudiffMax"]))
+    rec = gpu.moves_log(0, int(z["nmoves"]))
+    for f in FIELDS + ("trace_sign", "det_sign", "cfgsign"):
+        ref = z["move_" + f].astype(rec[f].dtype)
+        bad = np.nonzero(rec[f] != ref)[0]
+        assert len(bad) == 0, "%s: %s differs first at move %d" % (name, f, bad[0])
+    ev = int(z["every"])
+    for f in ("trace_log", "det_log"):
+        d = np.abs(rec[f][ev - 1::ev] - z["move_" + f])
+        assert d.max() <= TOL_LOG, (name, f, d.max())
+    gpu.close()
+
+
+def test_c4_short_golden_log():
+    """The 5-orbital single-window reference log (blocks up to 100 states)."""
+    z = load_golden("c4_log_w1")
+    q = qmc_of(z)
+    gpu = _golden_problem(z)
+    gpu.set_taudiffmax(float(q["TaudiffMax"]))
+    n = int(z["nmoves"])
+    rec = gpu.moves_log(0, n)
+    ref = {f: z["move_" + f][:n] for f in FIELDS + ("trace_log", "det_log", "trace_sign",
+                                                    "det_sign", "cfgsign")}
+    _compare(rec, ref, "c4_log_w1")
+    gpu.close()
+
+
+def _delta(prob, cfg, f):
+    """Hybridisation matrix Delta_f of a configuration (rows creators, columns
+    annihilators in time order, entries F(tau_a - tau_c)), as the reference
+    builds it (Trace.F90:2279-2333); get_minv returns M with M^T Delta = 1."""
+    nb = prob.nbands
+    b, s = f % nb, f // nb
+    sel = (cfg["orbs"] == b) & (cfg["spins"] == s)
+    tc = cfg["taus"][sel & (cfg["cas"] == 1)]
+    ta = cfg["taus"][sel & (cfg["cas"] == 2)]
+    F = prob.ftau[b, s, b, s]
+    beta = prob.beta
+
+    def hyb(t):
+        sg = np.where(t < 0, -1.0, 1.0)
+        t = np.where(t < 0, t + beta, t)
+        x = t / beta * (len(F) - 1)
+        i = np.minimum(x.astype(int), len(F) - 2)
+        return sg * (F[i] + (x - i) * (F[i + 1] - F[i]))
+
+    return hyb(ta[None, :] - tc[:, None])
+
+
+BATCH = {
+    # name: (problem, chains, capacity, warm-up moves per chain)
+    "c2": (lambda: P.kanamori_2orb(beta=50.0, TaudiffMax=5.0), 1024, 128, 20000),
+    "c3": (lambda: P.kanamori_3orb(beta=100.0, TaudiffMax=8.0), 4096, 384, 4000),
+    "c4": (lambda: P.coulomb_5orb(beta=40.0, TaudiffMax=5.0), 2048, 64, 600),
+}
+
+
+@pytest.mark.parametrize("case", list(BATCH))
+def test_full_batch_invariants(case):
+    mk, nch, cap, nwarm = BATCH[case]
+    prob = mk()
+    tdm = float(prob.qmc()["TaudiffMax"])
+    print("full batch", case, nch, "chains", nwarm, "moves each", flush=True)
+    gpu = CtqmcGpu.from_problem(prob, nchains=nch, seed=777, capacity=cap)
+    gpu.set_taudiffmax(tdm)
+    gpu.warmup(nwarm)
+    st = gpu.chain_status()
+    assert np.all(st["status"] == 0)
+    ora = OracleChain.from_problem(prob, 1)
+    orders = []
+    for c in np.linspace(0, nch - 1, 16).astype(int):
+        cfg = gpu.get_config(int(c))
+        orders.append(cfg["n"])
+        for f in range(2 * prob.nbands):
+            m = gpu.get_minv(int(c), f)
+            if m.size:
+                d = _delta(prob, cfg, f)
+                err = np.abs(m.T @ d - np.eye(len(m))).max()
+                assert err <= 1e-8, (case, c, f, err)
+        ref = ora.set_config(cfg["taus"], cfg["orbs"], cfg["spins"], cfg["cas"], cfg["outer_sst"])
+        for k in ("trace_log", "det_log"):
+            assert abs(cfg[k] - ref[k]) <= TOL_LOG * max(1.0, abs(ref[k]) / 100), (case, c, k, cfg[k], ref[k])
+        for k in ("trace_sign", "det_sign", "cfgsign"):
+            assert cfg[k] == ref[k], (case, c, k)
+    print("orders of the sampled chains", orders, flush=True)
+    assert np.mean(orders) > 20
+    gpu.close()
+
+
+MEAS = {
+    # 9-state blocks, 100 Legendre orders, > 8 pairs per flavour
+    "c3_leg100": (lambda: P.kanamori_3orb(beta=50.0, TaudiffMax=5.0, Nmeas=40, NCorr=100, Ntau=200,
+                                          NLegMax=100), 20000),
+    # more than 128 Legendre orders
+    "c3_leg150": (lambda: P.kanamori_3orb(beta=50.0, TaudiffMax=5.0, Nmeas=20, NCorr=100, Ntau=100,
+                                          NLegMax=150), 20000),
+    # 100-state blocks (FP64 MFMA walks, density matrix of blocks above 16 states)
+    "c4_leg100": (lambda: P.coulomb_5orb(beta=10.0, TaudiffMax=3.0, Nmeas=20, NCorr=50, Ntau=100,
+                                         NLegMax=100), 1500),
+}
+
+
+@pytest.mark.parametrize("case", list(MEAS))
+def test_measurements_match_oracle_large_blocks(case):
+    mk, nwarm = MEAS[case]
+    prob = mk()
+    q = prob.qmc()
+    tdm, nmeas, ncorr = float(q["TaudiffMax"]), int(q["Nmeas"]), int(q["NCorr"])
+    seed = 808
+    print("measurement case", case, flush=True)
+    gpu = CtqmcGpu.from_problem(prob, nchains=2, seed=seed, capacity=128)
+    gpu.set_taudiffmax(tdm)
+    gpu.warmup(nwarm)
+    gpu.measure(nmeas)
+    res = gpu.results()
+    for c in range(2):
+        ora = OracleChain.from_problem(prob, seed + c)
+        ora.set_taudiffmax(tdm)
+        ora.warmup(nwarm)
+        ora.measure(nmeas, ncorr)
+        k_now = max(len(ora.minv(f)) for f in range(2 * prob.nbands))
+        if c == 0 and case.startswith("c3"):
+            assert k_now > 8, "case does not reach the wide G(tau) passes"
+        r = ora.results()
+        for k in ("mean_sign", "gtau", "gleg", "occ", "densitymatrix", "histo", "acc", "outer_sst_histo"):
+            a, b = np.asarray(res[k][c]), np.asarray(r[k])
+            assert a.shape == b.shape, k
+            assert np.allclose(a, b, rtol=1e-9, atol=1e-11, equal_nan=True), (case, c, k, np.abs(a - b).max())
+    gpu.close()
+
+
+def _anderson_ed_gtau(eps_d, u, eps_k, v_k, beta, ntau):
+    """Exact bin averages of G_up(tau) = -<c(tau) c+> over the Ntau bins of
+    width beta/Ntau, from the full Fock space of impurity + bath."""
+    nbath = len(eps_k)
+    nso = 2 + 2 * nbath
+    dim = 1 << nso
+
+    def sgn(state, m):
+        return (-1) ** bin(state & ((1 << m) - 1)).count("1")
+
+    h = np.zeros((dim, dim))
+    for st in range(dim):
+        nu, nd = st & 1, (st >> 1) & 1
+        e = eps_d * (nu + nd) + u * nu * nd
+        for k in range(nbath):
+            for s in range(2):
+                e += eps_k[k] * ((st >> (2 * (k + 1) + s)) & 1)
+        h[st, st] = e
+        for k in range(nbath):
+            for s in range(2):
+                for a, b in ((s, 2 * (k + 1) + s), (2 * (k + 1) + s, s)):
+                    if (st >> b) & 1 and not ((st ^ (1 << b)) >> a) & 1:
+                        t = st ^ (1 << b)
+                        h[t ^ (1 << a), st] += v_k[k] * sgn(st, b) * sgn(t, a)
+    w, vec = np.linalg.eigh(h)
+    w = w - w[0]
+    c = np.zeros((dim, dim))   # c_up in the Fock basis
+    for st in range(dim):
+        if st & 1:
+            c[st ^ 1, st] = 1.0
+    ce = vec.T @ c @ vec      # <m|c|n>
+    z = np.exp(-beta * w).sum()
+    edges = np.linspace(0.0, beta, ntau + 1)
+    g = np.zeros(ntau)
+    em, en = np.meshgrid(w, w, indexing="ij")
+    a = em - en
+    wgt = np.exp(-beta * em) * ce ** 2
+    for i in range(ntau):
+        t0, t1 = edges[i], edges[i + 1]
+        with np.errstate(divide="ignore", invalid="ignore"):
+            integ = np.where(np.abs(a) > 1e-12, (np.exp(a * t1) - np.exp(a * t0)) / np.where(a == 0, 1, a),
+                             t1 - t0)
+        g[i] = -(wgt * integ).sum() / z / (t1 - t0)
+    return g
+
+
+def test_batched_chains_match_exact_diagonalisation():
+    """Impurity + 3 bath sites: G(tau) (bin averages), <n_up> and double
+    occupancy from 1024 batched chains lie within 3 sigma (jackknife over
+    groups of chains) of exact diagonalisation."""
+    beta, u, mu, ntau = 5.0, 2.0, 0.6, 25
+    eps_k, v_k = np.array([-1.0, 0.2, 1.1]), np.array([0.5, 0.35, 0.45])
+    p = P.anderson_1orb(beta=beta, u=u, nftau=4001, Ntau=ntau, NCorr=20, Nmeas=2000)
+    p.ftau = P.discrete_bath_ftau(beta, eps_k, v_k, 4001)
+    p.muimp = P.solver_muimp(1, mu)
+    n_ed, d_ed = _anderson_ed(-mu, u, eps_k, v_k, beta)
+    g_ed = _anderson_ed_gtau(-mu, u, eps_k, v_k, beta, ntau)
+    gpu = CtqmcGpu.from_problem(p, nchains=1024, seed=4000)
+    gpu.warmup(5000)
+    gpu.measure(2000)
+    agg = gpu.aggregate(ngroups=64)
+    gpu.close()
+    # discretisation of F(tau) on 4001 points limits the agreement to ~1e-4
+    for val, err, exact, what in ((agg["occ"][0, 0, 0, 0], agg["occ_err"][0, 0, 0, 0], n_ed, "n_up"),
+                                  (agg["occ"][0, 0, 0, 1], agg["occ_err"][0, 0, 0, 1], d_ed, "docc")):
+        print(what, val, "+-", err, "exact", exact, flush=True)
+        assert abs(val - exact) <= 3 * err + 2e-4, (what, val, err, exact)
+    # the reference's G(tau) is -<T c(tau) c+> with its sign reversed (positive)
+    g, ge = agg["gtau"][0, 0], agg["gtau_err"][0, 0]
+    dev = np.abs(g + g_ed) / (ge + 1e-4)
+    print("G(tau) largest deviation in sigma", dev.max(), flush=True)
+    assert np.mean(dev <= 3) >= 0.97 and dev.max() <= 4.5
+
+
+def test_batched_chains_match_reference_runs():
+    """G(tau), Legendre G, occupations and sign of 512 batched chains against
+    the mean of 16 independent reference runs (3 sigma of the combined error)."""
+    z = load_golden("c2_refstat")
+    q = qmc_of(z)
+    gpu = CtqmcGpu(str(z["param_string"]), z["umatrix"], z["ftau"], z["muimp"], nchains=512, seed=9000)
+    gpu.set_taudiffmax(float(q["TaudiffMax"]))
+    gpu.warmup(int(z["nwarm"]))
+    gpu.measure(int(q["Nmeas"]) // 4)
+    agg = gpu.aggregate(ngroups=64)
+    gpu.close()
+    for k in ("gtau", "gleg", "occ", "mean_sign"):
+        a, ae = np.asarray(agg[k]), np.asarray(agg[k + "_err"])
+        b, be = np.asarray(z["mean_" + k]), np.asarray(z["err_" + k])
+        sig = np.sqrt(ae ** 2 + be ** 2) + 1e-12
+        dev = np.abs(a - b) / sig
+        print(k, "largest deviation in sigma", float(dev.max()), flush=True)
+        assert np.mean(dev <= 3) >= 0.97 and dev.max() <= 4.5, (k, float(dev.max()))

@@ -7,6 +7,8 @@
 // host side: the C-ABI (include/w2dyn_ctqmc.h)
 // ===========================================================================
 #include <algorithm>
+#include <atomic>
+#include <csignal>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -34,6 +36,13 @@ struct w2d_ctqmc {
     std::vector<void *> dev_allocs;
     char *chains = nullptr;
     bool measured = false;
+    // abort word: pinned host memory the sweep kernel polls (request_abort)
+    int *abort_host = nullptr;
+    int *abort_dev = nullptr;
+    int aborted = 0;           // of the last warmup / measure (CTQMC.F90:8648-8658)
+    int regrows = 0;           // capacity doublings so far
+    bool grow = true;          // capacity doubles on demand (create's capacity >= 0)
+    ~w2d_ctqmc();
 };
 
 namespace {
@@ -63,6 +72,36 @@ int upload(w2d_ctqmc *h, const std::vector<T> &v, const T **out)
 }
 
 long align256(long x) { return (x + 255) / 256 * 256; }
+
+constexpr int KCAP_MAX = 4096;   // pairs per flavour a chain may grow to
+
+// abort words of the live handles, raised together by the signal handler
+constexpr int MAX_TRAPPED = 64;
+std::atomic<int *> g_abort_words[MAX_TRAPPED];
+
+void register_abort_word(int *w)
+{
+    for (auto &slot : g_abort_words) {
+        int *expect = nullptr;
+        if (slot.compare_exchange_strong(expect, w)) return;
+    }
+}
+
+void unregister_abort_word(int *w)
+{
+    for (auto &slot : g_abort_words) {
+        int *expect = w;
+        if (slot.compare_exchange_strong(expect, nullptr)) return;
+    }
+}
+
+extern "C" void w2d_signal_handler(int)
+{
+    for (auto &slot : g_abort_words) {
+        int *w = slot.load();
+        if (w) __atomic_store_n(w, 1, __ATOMIC_SEQ_CST);
+    }
+}
 
 void seed_mt(uint32_t *mt, uint32_t s)
 {
@@ -97,11 +136,13 @@ int nbound_for(double tdm, double beta)
     return std::max(nw, 1) + 2;
 }
 
-int layout_and_alloc(w2d_ctqmc *h, int nbound)
+// Byte layout of a chain block and of its LDS for `nbound` window boundaries
+// and `kc` pairs per flavour.
+int compute_layout(const w2d_ctqmc *h, int nbound, int kc, w2d::ChainLayout &L)
 {
     using namespace w2d;
-    ChainLayout &L = h->L;
-    const int nf = 2 * h->model.nbands, nm = h->nm, kc = h->kcap;
+    L = ChainLayout{};
+    const int nf = 2 * h->model.nbands, nm = h->nm;
     L.kcap = kc;
     L.cap = 2 * nf * kc;
     L.lcap = L.cap + 2;
@@ -202,15 +243,55 @@ int layout_and_alloc(w2d_ctqmc *h, int nbound)
     L.l_ops = (nm <= 4 || q + opsb <= 64 * 1024) ? lds(opsb) : -1;
     L.lds_bytes = q;
     if (L.lds_bytes > h->lds_max) return W2D_ERR_CAPACITY;
+    L.hdrs = L.stride * h->nchains;
+    return W2D_OK;
+}
+
+size_t chains_bytes(const w2d_ctqmc *h, const w2d::ChainLayout &L)
+{
+    return (size_t)L.hdrs + sizeof(ChainHdr) * (size_t)h->nchains;
+}
+
+int layout_and_alloc(w2d_ctqmc *h, int nbound)
+{
+    w2d::ChainLayout L;
+    int st = compute_layout(h, nbound, h->kcap, L);
+    if (st) return st;
     if (h->chains) {
         HIPCHK(hipFree(h->chains));
         h->chains = nullptr;
     }
-    L.hdrs = L.stride * h->nchains;
-    const size_t total = (size_t)L.hdrs + sizeof(ChainHdr) * (size_t)h->nchains;
+    h->L = L;
+    const size_t total = chains_bytes(h, L);
     HIPCHK(hipMalloc(&h->chains, total));
     HIPCHK(hipMemset(h->chains, 0, total));
     h->nwin_alloc = nbound;
+    return W2D_OK;
+}
+
+// Double the pairs-per-flavour capacity of every chain, keeping each chain's
+// state (a chain halted for room continues where it stopped).
+int regrow(w2d_ctqmc *h)
+{
+    const int kc = h->kcap * 2;
+    if (!h->grow || kc > KCAP_MAX) return W2D_ERR_CAPACITY;
+    w2d::ChainLayout B;
+    int st = compute_layout(h, h->L.nbound, kc, B);
+    if (st) return st;
+    char *nc = nullptr;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipMalloc(&nc, chains_bytes(h, B)));
+    HIPCHK(hipMemsetAsync(nc, 0, chains_bytes(h, B), h->stream));
+    const w2d::ChainLayout &A = h->L;
+    hipLaunchKernelGGL(w2d::migrate_kernel, dim3(h->nchains), dim3(w2d::WAVE), 0, h->stream, A, B,
+                       (const char *)h->chains, nc, 2 * h->model.nbands, h->nm, (long)(A.stride - A.gtau));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipFree(h->chains));
+    h->chains = nc;
+    h->L = B;
+    h->kcap = kc;
+    h->regrows++;
     return W2D_OK;
 }
 
@@ -232,11 +313,12 @@ void launch_init(w2d_ctqmc *h)
 }
 
 template <int NM>
-void launch_sweep(w2d_ctqmc *h, long long nmoves, long long ncorr, int only, w2d_move_record *rec)
+void launch_sweep(w2d_ctqmc *h, long long target, long long ncorr, int only, w2d_move_record *rec,
+                  long long rec_base)
 {
     allow_lds(w2d::sweep_kernel<NM>, h);
     hipLaunchKernelGGL(w2d::sweep_kernel<NM>, dim3(h->nchains), dim3(w2d::WAVE), (unsigned)h->L.lds_bytes, h->stream, h->dm, h->L,
-                       h->chains, nmoves, ncorr, only, rec);
+                       h->chains, target, ncorr, only, rec, rec_base, (const int *)h->abort_dev);
 }
 
 template <int NM>
@@ -309,12 +391,15 @@ int init_chains(w2d_ctqmc *h)
     return first_error(hd) ? W2D_ERR_ZERO_TRACE : W2D_OK;
 }
 
+// init_counters (CTQMC.F90:847) and the start of a warmup / measure call
 void reset_counters(ChainHdr &x)
 {
     x.try_add = x.try_rem = x.try_glob = x.try_shift = 0;
     x.acc_add = x.acc_rem = x.acc_glob = x.acc_shift = 0;
     x.qn_rej = x.moves = 0;
     x.alg_flops = x.alg_bytes = 0.0;
+    x.pos = 0;
+    x.halt = w2d::HALT_NONE;
 }
 
 int map_err(int e)
@@ -328,30 +413,109 @@ int map_err(int e)
     }
 }
 
+// Bring every chain to `target` moves of the current call in launches of at
+// most `per` moves (whole measurement intervals), starting from position
+// `from`.  Asynchronous: the caller synchronises.
+void launch_range(w2d_ctqmc *h, long long from, long long target, long long ncorr, long long per, int only,
+                  w2d_move_record *rec, long long rec_base)
+{
+    for (long long t = from; t < target;) {
+        t = std::min(target, t + per);
+        DISPATCH(launch_sweep, h, t, ncorr, only, rec, rec_base);
+        ++h->last_launches;
+    }
+}
+
+// After a pass: chains halted for room get a larger capacity and continue;
+// returns whether another pass is needed.  Chains that cannot grow further
+// are stopped with W2D_ERR_CAPACITY (their results are left out).
+int resume_halted(w2d_ctqmc *h, std::vector<ChainHdr> &hd, long long &from, bool &again)
+{
+    again = false;
+    bool cap = false;
+    from = -1;
+    for (const auto &x : hd) {
+        if (x.halt == w2d::HALT_ABORT) h->aborted = 1;
+        if (x.halt == w2d::HALT_CAP && x.err == 0) {
+            cap = true;
+            from = from < 0 ? x.pos : std::min(from, x.pos);
+        }
+    }
+    if (!cap || h->aborted) return W2D_OK;
+    const int st = regrow(h);
+    if (st && st != W2D_ERR_CAPACITY) return st;
+    for (auto &x : hd)
+        if (x.halt == w2d::HALT_CAP) {
+            x.halt = w2d::HALT_NONE;
+            if (st == W2D_ERR_CAPACITY) x.err = W2D_ERR_CAPACITY;
+        }
+    int wst = write_headers(h, hd);
+    if (wst) return wst;
+    again = (st == W2D_OK);
+    return W2D_OK;
+}
+
+// W2D_OK if every chain is fine, W2D_ERR_PARTIAL if some chains stopped on an
+// error and others did not, else the (first) error of the chains
+int batch_status(const std::vector<ChainHdr> &hd)
+{
+    int bad = 0, first = 0;
+    for (const auto &x : hd)
+        if (x.err) {
+            ++bad;
+            if (!first) first = x.err;
+        }
+    if (!bad) return W2D_OK;
+    return bad < (int)hd.size() ? W2D_ERR_PARTIAL : map_err(first);
+}
+
 // run nmoves steps per chain in launches of whole measurement intervals
 int run_moves(w2d_ctqmc *h, long long nmoves, long long ncorr)
 {
     h->moved = true;
+    h->aborted = 0;
     long long per = ncorr > 0 ? ncorr * std::max<long long>(1, 4000 / ncorr) : 4000;
-    HIPCHK(hipEventRecord(h->ev0, h->stream));
     h->last_launches = 0;
-    for (long long done = 0; done < nmoves; done += per, ++h->last_launches) {
-        long long n = std::min(per, nmoves - done);
-        DISPATCH(launch_sweep, h, n, ncorr, -1, nullptr);
-        HIPCHK(hipGetLastError());
-    }
-    HIPCHK(hipEventRecord(h->ev1, h->stream));
-    HIPCHK(hipEventSynchronize(h->ev1));
-    float ms = 0.f;
-    HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
-    h->last_ms = ms;
+    h->last_ms = 0.0;
     std::vector<ChainHdr> hd;
-    int st = read_headers(h, hd);
-    if (st) return st;
-    return map_err(first_error(hd));
+    long long from = 0;
+    for (bool again = true; again;) {
+        HIPCHK(hipEventRecord(h->ev0, h->stream));
+        launch_range(h, from, nmoves, ncorr, per, -1, nullptr, 0);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(h->ev1, h->stream));
+        HIPCHK(hipEventSynchronize(h->ev1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+        h->last_ms += ms;
+        int st = read_headers(h, hd);
+        if (st) return st;
+        if ((st = resume_halted(h, hd, from, again))) return st;
+    }
+    if (h->aborted) {
+        long long done = 0;
+        for (const auto &x : hd) done += x.cnt_meas;
+        if (ncorr > 0 && done == 0) h->aborted = 2;
+    }
+    return batch_status(hd);
 }
 
 }  // namespace
+
+// every resource the handle holds (also on the error paths of create)
+w2d_ctqmc::~w2d_ctqmc()
+{
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (void *p : dev_allocs) (void)hipFree(p);
+    if (chains) (void)hipFree(chains);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+    if (abort_host) {
+        unregister_abort_word(abort_host);
+        (void)hipHostFree(abort_host);
+    }
+}
 
 extern "C" {
 
@@ -378,8 +542,9 @@ int w2d_ctqmc_create(const char *params, int nbands, int nftau, const double *u_
     h->nchains = nchains;
     h->seed0 = seed0;
     h->device = device;
-    h->kcap = capacity > 0 ? capacity : 256;
-    if (h->kcap < 4 || h->kcap > 4096) return W2D_ERR_ARG;
+    h->kcap = capacity != 0 ? std::abs(capacity) : 256;
+    h->grow = capacity >= 0;
+    if (h->kcap < 2 || h->kcap > KCAP_MAX) return W2D_ERR_ARG;
     h->beta = P.real("beta");
     h->ntau = (int)P.integer("Ntau");
     h->nleg = (int)P.integer("NLegMax");
@@ -403,6 +568,11 @@ int w2d_ctqmc_create(const char *params, int nbands, int nftau, const double *u_
     HIPCHK(hipStreamCreate(&h->stream));
     HIPCHK(hipEventCreate(&h->ev0));
     HIPCHK(hipEventCreate(&h->ev1));
+    HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&h->abort_host), sizeof(int),
+                         hipHostMallocMapped | hipHostMallocCoherent));
+    *h->abort_host = 0;
+    HIPCHK(hipHostGetDevicePointer(reinterpret_cast<void **>(&h->abort_dev), h->abort_host, 0));
+    register_abort_word(h->abort_host);
     // device model
     w2d::DevModel &d = h->dm;
     const int nf = 2 * nbands;
@@ -417,8 +587,6 @@ int w2d_ctqmc_create(const char *params, int nbands, int nftau, const double *u_
     std::vector<double> fdiag((size_t)nf * nftau);
     for (int b = 0; b < nbands; ++b)
         for (int s = 0; s < 2; ++s) {
-            long base = b + 2L * nbands * 0 + 0;
-            (void)base;
             for (int t = 0; t < nftau; ++t) {
                 // ftau_full(b, s, b, s, t) in Fortran order (N,2,N,2,nftau)
                 long idx = b + (long)nbands * (s + 2L * (b + (long)nbands * (s + 2L * t)));
@@ -525,7 +693,7 @@ int w2d_ctqmc_measure(w2d_ctqmc *h, int64_t nmeas)
     HIPCHK(hipMemset2DAsync(h->chains + L.gtau, (size_t)L.stride, 0, (size_t)(L.stride - L.gtau),
                             (size_t)h->nchains, h->stream));
     st = run_moves(h, nmeas * h->ncorr, h->ncorr);
-    h->measured = (st == W2D_OK);
+    h->measured = (st == W2D_OK || st == W2D_ERR_PARTIAL);
     return st;
 }
 
@@ -620,20 +788,38 @@ int w2d_ctqmc_moves_log(w2d_ctqmc *h, int chain, int64_t n, w2d_move_record *rec
     if (h) h->moved = true;
     if (!h || chain < 0 || chain >= h->nchains || n < 0 || (!rec && n > 0)) return W2D_ERR_ARG;
     const long long per = 4000;
+    std::vector<ChainHdr> hd;
+    int st = read_headers(h, hd);
+    if (st) return st;
+    hd[chain].pos = 0;
+    hd[chain].halt = w2d::HALT_NONE;
+    if ((st = write_headers(h, hd))) return st;
     w2d_move_record *d = nullptr;
     HIPCHK(hipMalloc(&d, sizeof(w2d_move_record) * per));
-    int st = W2D_OK;
-    for (long long done = 0; done < n && st == W2D_OK; done += per) {
-        long long k = std::min<long long>(per, n - done);
-        DISPATCH(launch_sweep, h, k, 0, chain, d);
-        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(h->stream) != hipSuccess ||
-            hipMemcpy(rec + done, d, sizeof(w2d_move_record) * k, hipMemcpyDeviceToHost) != hipSuccess)
+    for (long long done = 0; done < n && st == W2D_OK;) {
+        const long long k = std::min<long long>(per, n - done);
+        DISPATCH(launch_sweep, h, done + k, 0, chain, d, done);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(h->stream) != hipSuccess) {
             st = W2D_ERR_HIP;
+            break;
+        }
+        if ((st = read_headers(h, hd))) break;
+        if (hd[chain].halt == w2d::HALT_CAP && hd[chain].err == 0) {
+            // grow and finish this chunk (records continue at pos - done)
+            long long from;
+            bool again;
+            if ((st = resume_halted(h, hd, from, again))) break;
+            if (again) continue;
+        }
+        if (hipMemcpy(rec + done, d, sizeof(w2d_move_record) * (hd[chain].pos - done), hipMemcpyDeviceToHost) !=
+            hipSuccess) {
+            st = W2D_ERR_HIP;
+            break;
+        }
+        if (hd[chain].err || hd[chain].halt) break;
+        done += k;
     }
     (void)hipFree(d);
-    if (st) return st;
-    std::vector<ChainHdr> hd;
-    st = read_headers(h, hd);
     if (st) return st;
     return map_err(hd[chain].err);
 }
@@ -685,7 +871,13 @@ int w2d_ctqmc_set_config(w2d_ctqmc *h, int chain, int n, const double *tau, cons
         cnt[2 * f + ca[i] - 1]++;
     }
     for (int f = 0; f < 2 * h->model.nbands; ++f)
-        if (cnt[2 * f] != cnt[2 * f + 1] || cnt[2 * f] >= h->kcap) return W2D_ERR_ARG;
+        if (cnt[2 * f] != cnt[2 * f + 1]) return W2D_ERR_ARG;
+    // grow until every flavour's order fits
+    for (int f = 0; f < 2 * h->model.nbands; ++f)
+        while (cnt[2 * f] > h->kcap) {
+            const int st = regrow(h);
+            if (st) return st;
+        }
     char *base = h->chains + (long)chain * h->L.stride;
     HIPCHK(hipStreamSynchronize(h->stream));
     ChainHdr x;
@@ -740,14 +932,113 @@ int w2d_ctqmc_last_kernel_ms(const w2d_ctqmc *h, double *ms, int64_t *launches)
 
 int w2d_ctqmc_destroy(w2d_ctqmc *h)
 {
-    if (!h) return W2D_OK;
-    if (h->stream) (void)hipStreamSynchronize(h->stream);
-    for (void *p : h->dev_allocs) (void)hipFree(p);
-    if (h->chains) (void)hipFree(h->chains);
-    if (h->ev0) (void)hipEventDestroy(h->ev0);
-    if (h->ev1) (void)hipEventDestroy(h->ev1);
-    if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
+    return W2D_OK;
+}
+
+int w2d_ctqmc_get_group_sums(w2d_ctqmc *h, int ngroups, double *gtau, double *gleg, double *dm_blocks,
+                             double *histo, double *outer, double *scalars, int32_t *nused)
+{
+    if (!h || ngroups < 1 || ngroups > h->nchains) return W2D_ERR_ARG;
+    if (!h->measured) return W2D_ERR_STATE;
+    const w2d::ChainLayout &L = h->L;
+    const int nf = 2 * h->model.nbands, nc = h->nchains;
+    std::vector<ChainHdr> hd;
+    int st = read_headers(h, hd);
+    if (st) return st;
+    std::vector<int> use(nc);
+    for (int c = 0; c < nc; ++c) use[c] = hd[c].err == 0;
+    const long nel = (L.stride - L.gtau) / 8;
+    int *duse = nullptr;
+    double *dout = nullptr;
+    HIPCHK(hipMalloc(&duse, (size_t)nc * sizeof(int)));
+    if (hipMalloc(&dout, (size_t)ngroups * nel * 8) != hipSuccess) {
+        (void)hipFree(duse);
+        return W2D_ERR_HIP;
+    }
+    std::vector<double> out((size_t)ngroups * nel);
+    bool ok = hipMemcpy(duse, use.data(), (size_t)nc * sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
+    if (ok) {
+        hipLaunchKernelGGL(w2d::group_sum_kernel, dim3((unsigned)((nel + 255) / 256), (unsigned)ngroups), dim3(256), 0,
+                           h->stream, (const char *)h->chains, L.stride, L.gtau, nel, nc, ngroups, duse, dout);
+        ok = hipGetLastError() == hipSuccess && hipStreamSynchronize(h->stream) == hipSuccess &&
+             hipMemcpy(out.data(), dout, out.size() * 8, hipMemcpyDeviceToHost) == hipSuccess;
+    }
+    (void)hipFree(duse);
+    (void)hipFree(dout);
+    if (!ok) return W2D_ERR_HIP;
+    auto put = [&](double *dst, long off, long n) {
+        if (!dst) return;
+        for (int g = 0; g < ngroups; ++g)
+            std::copy(out.begin() + (long)g * nel + (off - L.gtau) / 8,
+                      out.begin() + (long)g * nel + (off - L.gtau) / 8 + n, dst + (long)g * n);
+    };
+    put(gtau, L.gtau, (long)nf * h->ntau);
+    put(gleg, L.gleg, (long)nf * h->nleg);
+    put(dm_blocks, L.dm, h->dm.dmsize);
+    put(histo, L.histo, (long)nf * (h->maxhisto + 1));
+    put(outer, L.outer, h->model.nsst);
+    for (int g = 0; g < ngroups; ++g) {
+        const int c0 = (int)((long)g * nc / ngroups), c1 = (int)((long)(g + 1) * nc / ngroups);
+        double S[14] = {0};
+        int n = 0;
+        for (int c = c0; c < c1; ++c) {
+            if (!use[c]) continue;
+            const ChainHdr &x = hd[c];
+            const double v[14] = {x.meas_sign, (double)x.cnt_meas, (double)x.acc_add, (double)x.try_add,
+                                  (double)x.acc_rem, (double)x.try_rem, (double)x.acc_glob, (double)x.try_glob,
+                                  (double)x.acc_shift, (double)x.try_shift, (double)x.moves, (double)x.qn_rej,
+                                  x.alg_flops, x.alg_bytes};
+            for (int i = 0; i < 14; ++i) S[i] += v[i];
+            ++n;
+        }
+        if (scalars) std::copy(S, S + 14, scalars + 14L * g);
+        if (nused) nused[g] = n;
+    }
+    return W2D_OK;
+}
+
+int w2d_ctqmc_chain_status(const w2d_ctqmc *h, int32_t *status, int32_t *kfmax, int *capacity)
+{
+    if (!h) return W2D_ERR_ARG;
+    if (capacity) *capacity = h->kcap;
+    if (!status && !kfmax) return W2D_OK;
+    std::vector<ChainHdr> hd;
+    int st = read_headers(const_cast<w2d_ctqmc *>(h), hd);
+    if (st) return st;
+    const int nf = 2 * h->model.nbands;
+    for (int c = 0; c < h->nchains; ++c) {
+        if (status) status[c] = hd[c].err ? map_err(hd[c].err) : W2D_OK;
+        if (kfmax)
+            for (int f = 0; f < nf; ++f) kfmax[(long)c * nf + f] = hd[c].kfmax[f];
+    }
+    return W2D_OK;
+}
+
+int w2d_ctqmc_set_abort(w2d_ctqmc *h, int flag)
+{
+    if (!h) return W2D_ERR_ARG;
+    __atomic_store_n(h->abort_host, flag ? 1 : 0, __ATOMIC_SEQ_CST);
+    return W2D_OK;
+}
+
+int w2d_ctqmc_aborted(const w2d_ctqmc *h, int *aborted)
+{
+    if (!h || !aborted) return W2D_ERR_ARG;
+    *aborted = h->aborted;
+    return W2D_OK;
+}
+
+int w2d_ctqmc_trap_signals(void)
+{
+    const int sigs[3] = {SIGTERM, SIGINT, SIGUSR1};
+    for (int sg : sigs) {
+        struct sigaction sa;
+        std::memset(&sa, 0, sizeof(sa));
+        sa.sa_handler = w2d_signal_handler;
+        sigemptyset(&sa.sa_mask);
+        if (sigaction(sg, &sa, nullptr) != 0) return W2D_ERR_ARG;
+    }
     return W2D_OK;
 }
 

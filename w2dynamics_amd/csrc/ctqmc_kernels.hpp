@@ -1128,17 +1128,25 @@ DEV void contract(Ctx &c, const double *K, const double *B, int s, int no, doubl
 // Full trace of the current configuration (global moves, tau shifts,
 // set_config, initial state): identity ket on the outer superstate at tau = 0
 // through every operator, contracted with the identity bra at beta.
+// A configuration that breaks the quantum numbers (an operator without a
+// target block) has a zero trace: returns false with tl = -DBL_MAX, ts = 0.
 template <int NM>
-DEV void full_trace(Ctx &c, int outer, double &tl, double &ts)
+DEV bool full_trace(Ctx &c, int outer, double &tl, double &ts)
 {
     const int no = c.sz[outer];
     double *K = c.K0, *Kw = c.K1;
     set_identity(K, no);
     int s = outer;
     double lg = 0.0;
-    ket_walk_global<NM>(c, K, Kw, s, lg, no, 0.0, 0, c.h.nop, c.M.beta);
+    if (!ket_walk_global<NM>(c, K, Kw, s, lg, no, 0.0, 0, c.h.nop, c.M.beta) || s != outer) {
+        tl = -DBL_MAX;
+        ts = 0.0;
+        wsync();
+        return false;
+    }
     set_identity(Kw, no);
     contract(c, K, Kw, s, no, lg, tl, ts);
+    return true;
 }
 
 // ---------------------------------------------------------------------------
@@ -1721,9 +1729,11 @@ DEV int step_add(Ctx &c)
     const int f = orb + M.nb * spin;
     if (t2 == tref) return c.h.qn_rej++, OUT_QN;
     const int k = c.h.kf[f];
-    if (!ensure_window(c) || c.h.wib - c.h.wia + 2 > c.L.lcap_lds || k + 1 >= c.L.kcap ||
+    // room_for_move checked these before the move's first draw; a failure here
+    // is a logic error, not a full chain
+    if (!ensure_window(c) || c.h.wib - c.h.wia + 2 > c.L.lcap_lds || k + 1 > c.L.kcap ||
         c.h.nop + 2 > c.L.cap) {
-        c.h.err = ERR_CAP;
+        c.h.err = ERR_STATE_DEV;
         return OUT_REJ;
     }
     const int nw = c.h.wib - c.h.wia;
@@ -1818,6 +1828,7 @@ DEV int step_add(Ctx &c)
     }
     wsync();
     c.h.kf[f] = k + 1;
+    c.h.kfmax[f] = max(c.h.kfmax[f], k + 1);
     c.h.det_log += dr_log;
     c.h.det_sign *= dr_sign;
     c.h.trace_log = tl;
@@ -1844,7 +1855,7 @@ DEV int step_rem(Ctx &c)
     const DevModel &M = c.M;
     c.h.try_rem++;
     if (!ensure_window(c)) {
-        c.h.err = ERR_CAP;
+        c.h.err = ERR_STATE_DEV;
         return OUT_REJ;
     }
     const int nw = c.h.wib - c.h.wia;
@@ -2363,6 +2374,21 @@ DEV int step_shift(Ctx &c)
 
 // one Metropolis step with the dispatch of ctqmc_warmup / ctqmc_measure
 // (CTQMC.F90:8213-8254, 8432-8533) and the window-shift rule
+// Whether the next move fits the chain's memory whatever it proposes: every
+// flavour can take one more pair and the window plus a pair fits the LDS
+// lists.  Checked before the move's first random draw, so a chain that has no
+// room stops with its state and random stream intact and continues exactly
+// where it stopped once the host has enlarged its memory.  Loads the window
+// list (as the next insertion or removal would).
+DEV bool room_for_move(Ctx &c)
+{
+    int km = 0;
+    for (int f = 0; f < c.M.nf; ++f) km = max(km, c.h.kf[f]);
+    if (km + 1 > c.L.kcap || c.h.nop + 2 > c.L.cap) return false;
+    if (!ensure_window(c)) return false;
+    return c.h.wib - c.h.wia + 2 <= c.L.lcap_lds;
+}
+
 template <int NM>
 DEV void one_move(Ctx &c, int &kind, int &out)
 {
@@ -2751,12 +2777,19 @@ DEV void save_ctx(Ctx &c)
     wsync();
 }
 
-// Sweep kernel: each block advances chain blockIdx.x by `nmoves` steps; if
-// ncorr > 0 it measures after every ncorr steps.  rec != nullptr records
-// every step of chain `only` (parity mode; other chains return at once).
+// Sweep kernel: each block advances chain blockIdx.x until it has done
+// `target` moves in the current warmup / measure call (header field pos); if
+// ncorr > 0 it measures after every ncorr-th move of the call.  A chain stops
+// early, state intact, when the next move might not fit its memory
+// (HALT_CAP) or when the host has raised the abort word (HALT_ABORT, polled
+// at measurement boundaries and every 128 warm-up moves; the reference checks
+// its signal flag before every move, CTQMC.F90:8212, 8436).  rec != nullptr
+// records every step of chain `only` at rec[pos - rec_base] (parity mode;
+// other chains return at once).
 template <int NM>
-__global__ void __launch_bounds__(WAVE) sweep_kernel(DevModel M, ChainLayout L, char *chains, long long nmoves,
-                                                     long long ncorr, int only, w2d_move_record *rec)
+__global__ void __launch_bounds__(WAVE) sweep_kernel(DevModel M, ChainLayout L, char *chains, long long target,
+                                                     long long ncorr, int only, w2d_move_record *rec,
+                                                     long long rec_base, const int *abort_word)
 {
     const int chain = blockIdx.x;
     if (only >= 0 && chain != only) return;
@@ -2768,11 +2801,16 @@ __global__ void __launch_bounds__(WAVE) sweep_kernel(DevModel M, ChainLayout L, 
     double *dm = reinterpret_cast<double *>(base + L.dm);
     double *histo = reinterpret_cast<double *>(base + L.histo);
     double *ohist = reinterpret_cast<double *>(base + L.outer);
-    for (long long m = 0; m < nmoves && c.h.err == 0; ++m) {
+    while (c.h.pos < target && c.h.err == 0 && c.h.halt == HALT_NONE) {
+        if (!room_for_move(c)) {
+            c.h.halt = HALT_CAP;
+            break;
+        }
         int kind, out;
         one_move<NM>(c, kind, out);
+        const long long m = c.h.pos++;
         if (rec && lane_id() == 0) {
-            w2d_move_record &r = rec[m];
+            w2d_move_record &r = rec[m - rec_base];
             r.kind = kind;
             r.outcome = out;
             r.noper = c.h.nop;
@@ -2784,7 +2822,10 @@ __global__ void __launch_bounds__(WAVE) sweep_kernel(DevModel M, ChainLayout L, 
             r.det_sign = c.h.det_sign;
             r.cfgsign = c.h.cfgsign;
         }
-        if (ncorr > 0 && (m + 1) % ncorr == 0) {
+        if (ncorr <= 0) {
+            if ((m + 1) % 128 == 0 && __hip_atomic_load(abort_word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM))
+                c.h.halt = HALT_ABORT;
+        } else if ((m + 1) % ncorr == 0) {
             c.h.cnt_meas++;
             meas_misc(c, histo, ohist);
             if (M.meas_dm) meas_dm<NM>(c, dm);
@@ -2796,9 +2837,70 @@ __global__ void __launch_bounds__(WAVE) sweep_kernel(DevModel M, ChainLayout L, 
             const double2 wk = NM == 1 ? meas_gtau_narrow(ga) : meas_gtau(ga);
             c.wflops += wk.x;
             c.wbytes += wk.y;
+            if (__hip_atomic_load(abort_word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)) c.h.halt = HALT_ABORT;
         }
     }
     save_ctx(c);
+}
+
+// Copy every chain from layout A to layout B (larger capacity): header, random
+// state, configuration, flavour lists, inverse blocks re-strided to B's
+// leading dimension, trace caches and accumulators.  Work buffers are not
+// copied (they hold nothing between moves).  Physical row / column slots stay
+// valid since they are below A's capacity.
+DEV void copy_words(char *dst, const char *src, long bytes)
+{
+    int *d = reinterpret_cast<int *>(dst);
+    const int *s = reinterpret_cast<const int *>(src);
+    for (long i = lane_id(); i < bytes / 4; i += WAVE) d[i] = s[i];
+}
+
+__global__ void __launch_bounds__(WAVE) migrate_kernel(ChainLayout A, ChainLayout B, const char *src, char *dst,
+                                                       int nf, int nm, long accbytes)
+{
+    const int chain = blockIdx.x;
+    const char *s = src + (long)chain * A.stride;
+    char *d = dst + (long)chain * B.stride;
+    copy_words(dst + B.hdrs + (long)chain * sizeof(ChainHdr), src + A.hdrs + (long)chain * sizeof(ChainHdr),
+               sizeof(ChainHdr));
+    copy_words(d + B.mt, s + A.mt, 624 * 4);
+    copy_words(d + B.tau, s + A.tau, (long)A.cap * 8);
+    copy_words(d + B.code, s + A.code, (long)A.cap * 4);
+    for (int f = 0; f < nf; ++f) {
+        copy_words(d + B.fc + (long)f * B.kcap * 8, s + A.fc + (long)f * A.kcap * 8, (long)A.kcap * 8);
+        copy_words(d + B.fa + (long)f * B.kcap * 8, s + A.fa + (long)f * A.kcap * 8, (long)A.kcap * 8);
+        copy_words(d + B.fcs + (long)f * B.kcap * 4, s + A.fcs + (long)f * A.kcap * 4, (long)A.kcap * 4);
+        copy_words(d + B.fas + (long)f * B.kcap * 4, s + A.fas + (long)f * A.kcap * 4, (long)A.kcap * 4);
+        const double *ms = reinterpret_cast<const double *>(s + A.minv) + (long)f * A.kcap * A.kcap;
+        double *md = reinterpret_cast<double *>(d + B.minv) + (long)f * B.kcap * B.kcap;
+        for (long e = lane_id(); e < (long)A.kcap * A.kcap; e += WAVE) {
+            const long i = e % A.kcap, j = e / A.kcap;
+            md[i + j * B.kcap] = ms[e];
+        }
+    }
+    copy_words(d + B.kst, s + A.kst, (long)nm * nm * 8);
+    copy_words(d + B.bnd, s + A.bnd, (long)A.nbound * nm * nm * 8);
+    copy_words(d + B.bndlog, s + A.bndlog, (long)A.nbound * 8);
+    copy_words(d + B.bndsst, s + A.bndsst, (long)A.nbound * 4);
+    copy_words(d + B.part, s + A.part, 2L * nm * 8 + 16);
+    copy_words(d + B.gtau, s + A.gtau, accbytes);
+}
+
+// Accumulators (the contiguous tail [gtau, stride) of each chain block, as
+// doubles) summed over contiguous groups of chains: group g holds chains
+// [g*nc/ng, (g+1)*nc/ng) whose use[c] is set.  Grid: x over elements, y over
+// groups; consecutive threads read consecutive doubles of one chain.
+__global__ void __launch_bounds__(256) group_sum_kernel(const char *chains, long stride, long acc_off, long nel,
+                                                        int nc, int ng, const int *use, double *out)
+{
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    const int g = blockIdx.y;
+    if (e >= nel) return;
+    const int c0 = (int)((long)g * nc / ng), c1 = (int)((long)(g + 1) * nc / ng);
+    double acc = 0.0;
+    for (int c = c0; c < c1; ++c)
+        if (use[c]) acc += reinterpret_cast<const double *>(chains + (long)c * stride + acc_off)[e];
+    out[(long)g * nel + e] = acc;
 }
 
 // Initial state of every chain: empty configuration, initial outer superstate
@@ -2827,7 +2929,7 @@ __global__ void __launch_bounds__(WAVE) setconfig_kernel(DevModel M, ChainLayout
     c.h.wpos = 0;
     set_window(c);
     double tl, ts;
-    full_trace<NM>(c, c.h.outer, tl, ts);
+    if (!full_trace<NM>(c, c.h.outer, tl, ts)) c.h.err = 5;   // W2D_ERR_ZERO_TRACE
     c.h.trace_log = tl;
     c.h.trace_sign = ts;
     store_part(c, c.sz[c.h.outer]);

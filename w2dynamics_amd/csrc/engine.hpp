@@ -41,8 +41,15 @@ struct ChainHdr {
     long long acc_add, acc_rem, acc_glob, acc_shift;
     long long qn_rej, moves;
     double alg_flops, alg_bytes;   // algorithmic work of this chain's moves and measurements
+    long long pos;                 // moves done in the current warmup / measure call
+    int halt;                      // HALT_* : why the chain stopped before its target
+    int pad_;
     int kf[MAXF];
+    int kfmax[MAXF];               // largest order each flavour has reached
 };
+
+// reasons a chain stops before the launch's target position (chain state intact)
+enum { HALT_NONE = 0, HALT_CAP = 1, HALT_ABORT = 2 };
 
 // byte offsets of the per-chain arrays inside one chain block
 struct ChainLayout {

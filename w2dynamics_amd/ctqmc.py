@@ -27,6 +27,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import warnings
 
 import numpy as np
 
@@ -36,7 +37,8 @@ LIBNAME = os.path.join(HERE, "libw2dyn_ctqmc.so")
 W2D_ERRORS = {1: "invalid argument or unsupported parameter", 2: "HIP runtime error",
               3: "capacity exceeded (expansion order or window length)",
               4: "DetRat too small in removal", 5: "zero trace initial state",
-              6: "call out of order"}
+              6: "call out of order", 7: "some chains stopped on an error"}
+W2D_ERR_PARTIAL = 7
 
 
 class CtqmcError(RuntimeError):
@@ -76,6 +78,11 @@ def load_library(path: str | None = None):
     lib.w2d_ctqmc_get_minv.argtypes = [P, I, I, C.POINTER(I), P]
     lib.w2d_ctqmc_last_kernel_ms.argtypes = [P, C.POINTER(D), C.POINTER(C.c_int64)]
     lib.w2d_ctqmc_destroy.argtypes = [P]
+    lib.w2d_ctqmc_get_group_sums.argtypes = [P, I, P, P, P, P, P, P, P]
+    lib.w2d_ctqmc_chain_status.argtypes = [P, P, P, C.POINTER(I)]
+    lib.w2d_ctqmc_set_abort.argtypes = [P, I]
+    lib.w2d_ctqmc_aborted.argtypes = [P, C.POINTER(I)]
+    lib.w2d_ctqmc_trap_signals.argtypes = []
     lib.w2d_ctqmc_version.restype = C.c_char_p
     _lib = lib
     return lib
@@ -138,10 +145,41 @@ class CtqmcGpu:
         _check(self.lib.w2d_ctqmc_set_taudiffmax(self.h, float(x)), "w2d_ctqmc_set_taudiffmax")
 
     def warmup(self, n: int):
-        _check(self.lib.w2d_ctqmc_warmup(self.h, int(n)), "w2d_ctqmc_warmup")
+        self._check_batch(self.lib.w2d_ctqmc_warmup(self.h, int(n)), "w2d_ctqmc_warmup")
 
     def measure(self, nmeas: int):
-        _check(self.lib.w2d_ctqmc_measure(self.h, int(nmeas)), "w2d_ctqmc_measure")
+        self._check_batch(self.lib.w2d_ctqmc_measure(self.h, int(nmeas)), "w2d_ctqmc_measure")
+
+    def _check_batch(self, code: int, where: str):
+        """A batch where some chains stopped on an error carries on with the
+        others (their results are left out, ``chain_status`` names them)."""
+        if code == W2D_ERR_PARTIAL:
+            st = self.chain_status()["status"]
+            warnings.warn("%s: %d of %d chains stopped on an error (codes %s); their results are "
+                          "left out" % (where, int((st != 0).sum()), self.nchains,
+                                        sorted(set(st[st != 0].tolist()))), RuntimeWarning)
+            return
+        _check(code, where)
+
+    def chain_status(self) -> dict:
+        """Per-chain status (0 = ok, else the error that stopped it), the largest
+        order each flavour reached, and the current pairs-per-flavour capacity."""
+        st = np.zeros(self.nchains, np.int32)
+        km = np.zeros((self.nchains, 2 * self.nbands), np.int32)
+        cap = C.c_int()
+        _check(self.lib.w2d_ctqmc_chain_status(self.h, _p(st), _p(km), C.byref(cap)), "chain_status")
+        return dict(status=st, kfmax=km, capacity=cap.value)
+
+    def set_abort(self, flag: bool = True):
+        """Raise (or clear) the abort word: sweeps stop at the next measurement
+        boundary and keep what they measured (the reference's signal handling)."""
+        _check(self.lib.w2d_ctqmc_set_abort(self.h, 1 if flag else 0), "set_abort")
+
+    def aborted(self) -> int:
+        """``aborted`` of the last warmup/measure: 0, 1 (data kept) or 2 (none)."""
+        a = C.c_int()
+        _check(self.lib.w2d_ctqmc_aborted(self.h, C.byref(a)), "aborted")
+        return a.value
 
     def moves_log(self, chain: int, n: int) -> np.ndarray:
         rec = np.zeros(n, dtype=MOVE_DT)
@@ -223,24 +261,49 @@ class CtqmcGpu:
         return dict(gtau=g, gleg=gl, dm=dm, histo=hi, outer_hist=oh, scalars=sc)
 
     def results(self, acc: dict | None = None) -> dict:
-        """Per-chain observables normalised as in ctqmc_measure; leading axis = chain."""
+        """Per-chain observables normalised as in ctqmc_measure (each chain like
+        one reference rank); leading axis = chain.  A chain whose own sign sum
+        vanishes gives NaN; ``aggregate`` pools chains instead."""
         if acc is None:
             acc = self.accumulators()
         return normalise(acc, self.nbands, self.beta, self.ntau, self.nleg, self.model())
 
-    def aggregate(self, res: dict | None = None) -> dict:
-        """Mean and standard error over chains (DistributedSample semantics)."""
-        if res is None:
-            res = self.results()
-        out = {}
-        for k, v in res.items():
-            v = np.asarray(v, dtype=float)
-            n = v.shape[0]
-            mean = v.mean(0)
-            err = np.sqrt(((v - mean) ** 2).sum(0) / (n * (n - 1))) if n > 1 else np.full_like(mean, np.nan)
-            out[k] = mean
-            out[k + "_err"] = err
-        return out
+    def group_sums(self, ngroups: int) -> dict:
+        """Raw accumulators summed on the GPU over ``ngroups`` contiguous groups
+        of chains, leaving out chains stopped on an error; the density matrix
+        comes as its full (nstates x nstates) form per group."""
+        nf, ng = 2 * self.nbands, int(ngroups)
+        m = self.model()
+        dmsize = int((m["sizes"].astype(np.int64) ** 2).sum())
+        g = np.zeros((ng, nf, self.ntau))
+        gl = np.zeros((ng, nf, self.nleg))
+        db = np.zeros((ng, dmsize))
+        hi = np.zeros((ng, nf, self.maxhisto + 1))
+        oh = np.zeros((ng, self.nsst))
+        sc = np.zeros((ng, 14))
+        nu = np.zeros(ng, np.int32)
+        _check(self.lib.w2d_ctqmc_get_group_sums(self.h, ng, _p(g), _p(gl), _p(db), _p(hi), _p(oh),
+                                                 _p(sc), _p(nu)), "get_group_sums")
+        dm = np.zeros((ng, self.nstates, self.nstates))
+        o = 0
+        for off, n in zip(m["offsets"], m["sizes"]):
+            dm[:, off:off + n, off:off + n] = db[:, o:o + n * n].reshape(ng, n, n).transpose(0, 2, 1)
+            o += n * n
+        return dict(gtau=g, gleg=gl, dm=dm, histo=hi, outer_hist=oh, scalars=sc, nused=nu)
+
+    def aggregate(self, ngroups: int = 64) -> dict:
+        """Estimates over all chains that completed: the sign-ratio estimate
+        (sums of sign-weighted measurements over chains divided by the sum of
+        signs, i.e. CTQMC.F90:8663-8771 applied to pooled sums) and its
+        jackknife standard error over groups of chains (``<key>_err``).  Unlike
+        a mean of per-chain ratios it stays finite where a single chain's sign
+        sum vanishes; without a sign problem it agrees with the
+        DistributedSample mean (statistics.py:180-209) to O(1/chains)."""
+        ok = int((self.chain_status()["status"] == 0).sum())
+        ng = max(1, min(int(ngroups), ok))
+        sums = self.group_sums(ng)
+        sums = {k: v[sums["nused"] > 0] for k, v in sums.items()}
+        return pooled_estimate(sums, self.nbands, self.beta, self.ntau, self.nleg, self.model())
 
 
 def _param_value(params: str, key: str) -> float:
@@ -250,8 +313,38 @@ def _param_value(params: str, key: str) -> float:
     raise KeyError(key)
 
 
+def pooled_estimate(sums: dict, nb: int, beta: float, ntau: int, nleg: int, model: dict) -> dict:
+    """Normalised observables of the pooled sums over the leading (group) axis
+    of ``sums`` and their jackknife errors over that axis."""
+    keys = ("gtau", "gleg", "dm", "histo", "outer_hist", "scalars")
+    tot = {k: np.asarray(sums[k], dtype=float).sum(0)[None] for k in keys}
+    est = normalise(tot, nb, beta, ntau, nleg, model)
+    out = {k: np.asarray(v)[0] for k, v in est.items()}
+    n = len(sums["scalars"])
+    if n < 2:
+        for k in est:
+            out[k + "_err"] = np.full_like(out[k], np.nan, dtype=float)
+        return out
+    jk = {k: [] for k in est}
+    for i in range(n):
+        part = {k: tot[k] - np.asarray(sums[k], dtype=float)[i][None] for k in keys}
+        r = normalise(part, nb, beta, ntau, nleg, model)
+        for k in est:
+            jk[k].append(np.asarray(r[k], dtype=float)[0])
+    for k in est:
+        a = np.array(jk[k])
+        out[k + "_err"] = np.sqrt((n - 1) / n * ((a - a.mean(0)) ** 2).sum(0))
+    out["nchains_groups"] = n
+    return out
+
+
 def normalise(acc: dict, nb: int, beta: float, ntau: int, nleg: int, model: dict) -> dict:
     """Normalisation of CTQMC.F90:8663-8771 for every chain of ``acc``."""
+    with np.errstate(invalid="ignore", divide="ignore"):
+        return _normalise(acc, nb, beta, ntau, nleg, model)
+
+
+def _normalise(acc, nb, beta, ntau, nleg, model):
     sc = acc["scalars"]
     cnt = sc[:, 1]
     mean_sign = sc[:, 0] / cnt
@@ -274,9 +367,8 @@ def normalise(acc: dict, nb: int, beta: float, ntau: int, nleg: int, model: dict
     states = model["states"]
     nocc = np.array([[(states >> (b + nb * s)) & 1 for s in range(2)] for b in range(nb)])
     occ = np.einsum("ck,ijk,lmk->cijlm", diag, nocc, nocc)
-    with np.errstate(invalid="ignore", divide="ignore"):
-        accr = np.stack([sc[:, 2] / sc[:, 3], sc[:, 4] / sc[:, 5], sc[:, 6] / sc[:, 7],
-                         sc[:, 8] / sc[:, 9]], axis=1)
+    accr = np.stack([sc[:, 2] / sc[:, 3], sc[:, 4] / sc[:, 5], sc[:, 6] / sc[:, 7],
+                     sc[:, 8] / sc[:, 9]], axis=1)
     return dict(mean_sign=mean_sign, gtau=gtau, gleg=gleg, occ=occ, densitymatrix=dm_occ,
                 histo=fl2(acc["histo"]) / cnt[:, None, None, None], acc=accr,
                 outer_sst_histo=acc["outer_hist"] / cnt[:, None])
