@@ -1,17 +1,29 @@
 """Benchmark: CT-HYB Metropolis moves/s and G(tau) 1/err^2/s on one MI355X,
 batched chains vs the plain single-chain path.
 
-Workload (default): 2-orbital Kanamori impurity, U=4, J=1, U'=U-2J with spin flip
-and pair hopping, half filling, beta=50, semicircular baths of half-bandwidth 2,
-16 local states in (N,Sz) blocks, FP64; 1024 batched Markov chains.  A "step" is
-one sweep: NCorr Metropolis moves of every chain followed by the Z-space
-measurements (G(tau) bins + Legendre, beta/2 density matrix, sign, histograms).
+Workload (default, c4): 5-orbital full-Coulomb impurity (F0=4, J=0.8, F4/F2=0.625),
+filling 6, beta=40, seeded Gaussian crystal field, semicircular baths of
+half-bandwidth 2, 1024 local states in (N,Sz) blocks up to 100 x 100 (the local
+trace runs on the FP64 matrix cores), 2048 batched Markov chains, NCorr=100,
+TaudiffMax=5.  A "step" is one sweep: NCorr Metropolis moves of every chain
+followed by the Z-space measurements (G(tau) bins + Legendre, beta/2 density
+matrix, sign, histograms).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--chains 1024]
-                       [--config c2|c1|c3|c4] [--no-cpu] [--no-plain]
+Before the timed region the chains are thermalised until the mean expansion
+order over all chains stops growing (whatever --warmup says; --warmup is the
+minimum number of sweeps), and the mean order is reported before and after the
+timed region.  The plain single-chain path is one chain of the same engine,
+started from a thermalised configuration and measured with the binning
+accumulator (AccumGtau = 2); its G(tau) error is the binning plateau.  The CPU
+baseline is the reference Fortran core built from its sources (oracle/_ref),
+one process per host core, each started from a thermalised configuration
+(set_mc_config, as ReuseMCConfig does) under a fixed measurement budget.
 
-Prints one JSON line (see the "metric" keys) with a roofline object for the
-sweep kernel and the reference CPU path timed on this host as cpu_baseline.
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c4|c3|c2|c1]
+                       [--chains N] [--no-cpu] [--no-plain] [--therm-s S]
+
+Prints one JSON line with a roofline object for the sweep kernel (FP64 MFMA for
+blocks above 16 states, HBM otherwise) and the reference CPU path as cpu_baseline.
 """
 
 from __future__ import annotations
@@ -31,20 +43,21 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0      # MI355X HBM3E
 FP64_PEAK_TFS = 78.6       # MI355X dense FP64 (vector and matrix)
+METRIC = "MC moves/s and G(τ) 1/err²/s on 1 GPU, batched chains vs plain path"
 
 WORKLOADS = {
-    # name: (builder kwargs, TaudiffMax, chains, capacity)
-    "c1": ("anderson_1orb", dict(beta=10.0), 2.0, 1024, 128),
-    "c2": ("kanamori_2orb", dict(beta=50.0), 5.0, 1024, 128),
-    "c3": ("kanamori_3orb", dict(beta=100.0), 8.0, 4096, 384),
-    "c4": ("coulomb_5orb", dict(beta=40.0), 5.0, 2048, 64),
+    # name: (builder, builder kwargs, TaudiffMax, chains, initial capacity, reference Nmeas per process)
+    "c1": ("anderson_1orb", dict(beta=10.0), 2.0, 1024, 128, 20000),
+    "c2": ("kanamori_2orb", dict(beta=50.0), 5.0, 1024, 128, 20000),
+    "c3": ("kanamori_3orb", dict(beta=100.0), 8.0, 4096, 384, 300),
+    "c4": ("coulomb_5orb", dict(beta=40.0), 5.0, 2048, 64, 12),
 }
 
 
-def make_problem(cfg, ncorr):
+def make_problem(cfg, ncorr, **extra):
     from w2dynamics_amd import problems as P
-    name, kw, tdm, _, _ = WORKLOADS[cfg]
-    return getattr(P, name)(TaudiffMax=tdm, NCorr=ncorr, **kw), tdm
+    name, kw, tdm = WORKLOADS[cfg][:3]
+    return getattr(P, name)(TaudiffMax=tdm, NCorr=ncorr, **kw, **extra), tdm
 
 
 def gtau_inv_err2(res):
@@ -54,8 +67,7 @@ def gtau_inv_err2(res):
     m = g.mean(0)
     se2 = ((g - m) ** 2).sum(0) / (n * (n - 1))
     if not float(se2.mean()) > 0.0:
-        print("G(tau) samples carry no spread (sum |G| = %g); 1/err^2 undefined" % float(np.abs(g).sum()),
-              file=sys.stderr)
+        print("G(tau) samples carry no spread; 1/err^2 undefined", file=sys.stderr)
         return float("nan")
     return 1.0 / float(se2.mean())
 
@@ -63,9 +75,9 @@ def gtau_inv_err2(res):
 def ratio_inv_err2(graw, sgn, ntau, beta):
     """1 / <stderr^2(G(tau))> of the sign-ratio estimate G = sum G_raw / sum sign
     (the normalisation of CTQMC.F90:8663-8771 applied to pooled sums), by the
-    jackknife over the leading axis (blocks of chains or of measurements).
-    Without a sign problem it is the plain standard error of the block means;
-    with one it stays finite where a single block's sign sum vanishes."""
+    jackknife over the leading axis (groups of chains).  Without a sign problem
+    it is the plain standard error of the group means; with one it stays finite
+    where a single group's sign sum vanishes."""
     g = np.asarray(graw, dtype=float).reshape(len(sgn), -1)
     s = np.asarray(sgn, dtype=float)
     n = g.shape[0]
@@ -79,117 +91,200 @@ def ratio_inv_err2(graw, sgn, ntau, beta):
     return 1.0 / float(se2.mean())
 
 
-def gtau_inv_err2_chains(acc, ntau, beta, nblocks=64):
-    """Batched run: the chains pooled into `nblocks` contiguous blocks."""
-    g = np.asarray(acc["gtau"], dtype=float)            # [chain, flavour, ntau] raw sums
-    sg = np.asarray(acc["scalars"], dtype=float)[:, 0]  # sum of signs per chain
-    nb = min(nblocks, g.shape[0])
-    blocks = np.array_split(np.arange(g.shape[0]), nb)
-    return ratio_inv_err2(np.stack([g[i].sum(0) for i in blocks]), np.array([sg[i].sum() for i in blocks]),
-                          ntau, beta)
+def plateau_inv_err2(blocks, nmeas, mean_sign, ntau, beta, min_bins=16):
+    """Single chain: binning analysis of the raw G(tau) (gtau_blocks of the
+    autocorracc accumulator, Accumulators.F90:283-362).  Level l holds the
+    variance of the means of bins of 2^l measurements; stderr^2 of the mean at
+    level l is var_l / n_l with n_l = nmeas >> l.  The plateau is the largest
+    <stderr^2> over the levels with at least `min_bins` bins (conservative);
+    the normalisation of G (1/|mean sign| * ntau / beta^2) is applied."""
+    curve = []
+    for lvl in range(blocks.shape[0]):
+        nl = nmeas >> lvl
+        if nl < min_bins:
+            break
+        se2 = blocks[lvl] / nl * (ntau / beta ** 2 / abs(mean_sign)) ** 2
+        curve.append(float(np.mean(se2)))
+    if not curve or not max(curve) > 0:
+        return float("nan"), curve
+    return 1.0 / max(curve), curve
 
 
-def run_gpu(prob, tdm, nchains, capacity, warm_sweeps, steps, ncorr, seed):
+def thermalise(gpu, ncorr, min_sweeps, chunk, budget_s, tol=0.015):
+    """Warm up in chunks of `chunk` sweeps until the mean order over all chains
+    has stopped growing (the mean of the last two chunks within `tol` of the
+    two before) and at least `min_sweeps` sweeps are done, or `budget_s` is
+    spent.  Returns the trajectory of (sweeps, mean order, seconds)."""
+    traj = [(0, float(gpu.orders().mean()), 0.0)]
+    t0 = time.perf_counter()
+    sweeps = 0
+    while True:
+        gpu.warmup(chunk * ncorr)
+        sweeps += chunk
+        traj.append((sweeps, float(gpu.orders().mean()), time.perf_counter() - t0))
+        m = [x[1] for x in traj[1:]]
+        if sweeps >= min_sweeps and len(m) >= 4 and \
+                abs(np.mean(m[-2:]) - np.mean(m[-4:-2])) <= tol * max(1.0, np.mean(m[-2:])):
+            break
+        if time.perf_counter() - t0 > budget_s:
+            print("thermalisation budget of %.0f s spent before the order settled" % budget_s,
+                  file=sys.stderr)
+            break
+    return traj
+
+
+def run_gpu(cfg, nchains, capacity, min_sweeps, steps, ncorr, seed, therm_s):
     import torch
     from w2dynamics_amd.ctqmc import CtqmcGpu
+    prob, tdm = make_problem(cfg, ncorr)
     gpu = CtqmcGpu.from_problem(prob, nchains=nchains, seed=seed, capacity=capacity, device=0)
     gpu.set_taudiffmax(tdm)
-    gpu.warmup(warm_sweeps * ncorr)
+    chunk = 5 if cfg in ("c3", "c4") else 20
+    traj = thermalise(gpu, ncorr, min_sweeps, chunk, therm_s)
+    order_before = float(gpu.orders().mean())
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     gpu.measure(steps)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     kms, nlaunch = gpu.last_kernel_ms()
-    acc = gpu.accumulators()
-    res = gpu.results(acc)
+    order_after = float(gpu.orders().mean())
+    ng = min(64, nchains)
+    sums = gpu.group_sums(ng, with_dm=False)
+    sc = sums["scalars"]
+    q = prob.qmc()
     out = dict(wall=wall, kernel_ms=kms, launches=nlaunch,
-               inv_err2=gtau_inv_err2_chains(acc, prob.qmc()["Ntau"], prob.beta),
-               moves=float(acc["scalars"][:, 10].sum()),
-               alg_flops=float(acc["scalars"][:, 12].sum()),
-               alg_bytes=float(acc["scalars"][:, 13].sum()), res=res,
-               order=float(np.mean([gpu.get_config(c)["n"] for c in range(min(nchains, 8))])))
+               inv_err2=ratio_inv_err2(sums["gtau"], sc[:, 0], int(q["Ntau"]), prob.beta),
+               moves=float(sc[:, 10].sum()), alg_flops=float(sc[:, 12].sum()),
+               alg_bytes=float(sc[:, 13].sum()), mean_sign=float(sc[:, 0].sum() / sc[:, 1].sum()),
+               order_before=order_before, order_after=order_after, traj=traj,
+               status=gpu.chain_status(), nops_bytes=8.0 * gpu.model_ops_size(),
+               configs=[gpu.get_config(c) for c in range(min(nchains, 64))], label=prob.label,
+               beta=prob.beta, tdm=tdm)
     gpu.close()
     return out
 
 
-def run_plain(prob, tdm, capacity, warm_sweeps, steps, ncorr, seed, blocks=8):
-    """Single chain: the same engine with one chain; error from block means."""
+def run_plain(cfg, capacity, ncorr, seed, start_cfg, budget_s=25.0):
+    """Single chain from a thermalised configuration; binning-plateau error."""
     import torch
     from w2dynamics_amd.ctqmc import CtqmcGpu
+    # a short timed run sizes the measurement to the budget
+    prob, tdm = make_problem(cfg, ncorr, AccumGtau=2, Nmeas=4)
     gpu = CtqmcGpu.from_problem(prob, nchains=1, seed=seed, capacity=capacity, device=0)
     gpu.set_taudiffmax(tdm)
-    gpu.warmup(warm_sweeps * ncorr)
-    per = max(1, steps // blocks)
-    g, sg = [], []
+    c = start_cfg
+    gpu.set_config(0, c["taus"], c["orbs"], c["spins"], c["cas"], c["outer_sst"])
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(blocks):
-        gpu.measure(per)
-        acc = gpu.accumulators()
-        g.append(np.asarray(acc["gtau"][0], dtype=float))
-        sg.append(float(acc["scalars"][0, 0]))
+    gpu.measure(4)
+    torch.cuda.synchronize()
+    rate = 4 * ncorr / (time.perf_counter() - t0)
+    gpu.close()
+    nmeas = int(min(8192, max(64, budget_s * rate / ncorr)))
+    prob, tdm = make_problem(cfg, ncorr, AccumGtau=2, Nmeas=nmeas)
+    gpu = CtqmcGpu.from_problem(prob, nchains=1, seed=seed, capacity=capacity, device=0)
+    gpu.set_taudiffmax(tdm)
+    gpu.set_config(0, c["taus"], c["orbs"], c["spins"], c["cas"], c["outer_sst"])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    gpu.measure(nmeas)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    sc = gpu.group_sums(1, with_dm=False)["scalars"][0]
+    blocks = gpu.gtau_blocks(0)
+    q = prob.qmc()
+    inv, curve = plateau_inv_err2(blocks, nmeas, sc[0] / sc[1], int(q["Ntau"]), prob.beta)
+    order = float(gpu.orders().mean())
     gpu.close()
-    moves = blocks * per * ncorr
-    return dict(wall=wall, moves=moves,
-                inv_err2=ratio_inv_err2(np.array(g), np.array(sg), prob.qmc()["Ntau"], prob.beta))
+    return dict(wall=wall, moves=float(nmeas * ncorr), inv_err2=inv, nmeas=nmeas, curve=curve,
+                order=order)
 
 
-def cpu_baseline(prob, tdm, seed, nwarm, nmeas, ncorr):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def write_configs(path, cfg):
+    """configs.bin of the reference driver: int32 count, int32 n, int32 outer
+    superstate (0-based), n taus, then orbitals, spins (1-based) and c/a codes."""
+    with open(path, "wb") as f:
+        np.array([1, cfg["n"], cfg["outer_sst"]], "<i4").tofile(f)
+        np.asarray(cfg["taus"], "<f8").tofile(f)
+        (np.asarray(cfg["orbs"]) + 1).astype("<i4").tofile(f)
+        (np.asarray(cfg["spins"]) + 1).astype("<i4").tofile(f)
+        np.asarray(cfg["cas"]).astype("<i4").tofile(f)
+
+
+def cpu_baseline(cfg, ncorr, seed, configs):
     """The reference CPU path (oracle/_ref/ref_driver, built from the reference
-    sources) timed on this host: one process per core, seeds NSeed + i, like MPI
-    ranks; falls back to the C oracle (a single-threaded port) if the reference
-    build is absent."""
+    sources) on this host's cores: one process per core (seeds NSeed + i, like
+    MPI ranks), each started from a thermalised configuration of a GPU chain
+    and measuring a fixed number of sweeps.  Falls back to the C oracle (a
+    single-threaded port) if the reference build is absent."""
     sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
     drv = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
     try:
         ncpu = len(os.sched_getaffinity(0))
     except AttributeError:
         ncpu = os.cpu_count() or 1
-    nproc = max(1, min(16, ncpu))
-    p = prob
-    p.params = dict(p.params, Nmeas=nmeas, NCorr=ncorr)
+    nproc = max(1, min(16, ncpu, len(configs)))
+    nmeas = WORKLOADS[cfg][5]
+    prob, tdm = make_problem(cfg, ncorr, Nmeas=nmeas)
     if os.path.exists(drv):
         try:
             import make_golden as MG
             dirs = []
             for i in range(nproc):
                 d = tempfile.mkdtemp(prefix="w2dref")
-                MG.write_inputs(d, p, "run", seed + i, nwarm, 0)
+                MG.write_inputs(d, prob, "runcfg", seed + i, 0, 0)
+                write_configs(os.path.join(d, "configs.bin"), configs[i])
                 dirs.append(d)
             t0 = time.perf_counter()
             procs = [subprocess.Popen([drv, d], stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
                      for d in dirs]
-            rcs = [q.wait() for q in procs]
+            rcs = [p.wait() for p in procs]
             wall = time.perf_counter() - t0
             if all(rc == 0 for rc in rcs):
-                nb, ntau = p.nbands, int(p.qmc()["Ntau"])
-                g = []
+                nb, ntau = prob.nbands, int(prob.qmc()["Ntau"])
+                g, tsim, orders = [], [], []
                 for d in dirs:
                     raw = np.fromfile(os.path.join(d, "results.bin"), dtype="<f8")
                     g.append(raw[1:1 + nb * 2 * ntau].reshape((nb, 2, ntau), order="F"))
-                moves = nproc * (nwarm + nmeas * ncorr)
+                    tw, ts, nop = open(os.path.join(d, "timing.txt")).read().split()
+                    tsim.append(float(ts))
+                    orders.append(int(nop))
+                moves = nproc * nmeas * ncorr
                 return dict(value=moves / wall, unit="moves/s", cores=nproc, kind="reference",
-                            sample="%d processes x (%d warmup + %d x %d measured moves), %s" % (
-                                nproc, nwarm, nmeas, ncorr, p.label),
+                            cpu=cpu_model(),
+                            sample="%d processes x %d measured sweeps of %d moves from thermalised "
+                                   "configurations (mean order %.0f operators), %s" % (
+                                       nproc, nmeas, ncorr, np.mean([c["n"] for c in configs[:nproc]]),
+                                       prob.label),
                             gtau_inv_err2_per_s=gtau_inv_err2({"gtau": np.array(g)}) / wall,
-                            wall_s=wall)
+                            wall_s=wall, sampling_s_max=max(tsim), final_mean_order=float(np.mean(orders)))
+            print("reference runs returned %s" % rcs, file=sys.stderr)
         except Exception as e:  # reported, then fall back to the port
             print("reference CPU run failed (%s); timing the C oracle instead" % e, file=sys.stderr)
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from oracle import OracleChain
-    o = OracleChain.from_problem(p, seed)
+    o = OracleChain.from_problem(prob, seed)
     o.set_taudiffmax(tdm)
+    c = configs[0]
+    o.set_config(c["taus"], c["orbs"], c["spins"], c["cas"], c["outer_sst"])
     t0 = time.perf_counter()
-    o.warmup(nwarm)
     o.measure(nmeas, ncorr)
     wall = time.perf_counter() - t0
-    return dict(value=(nwarm + nmeas * ncorr) / wall, unit="moves/s", cores=1, kind="port",
-                sample="1 chain x (%d warmup + %d x %d measured moves), %s" % (nwarm, nmeas, ncorr,
-                                                                               p.label),
-                wall_s=wall)
+    return dict(value=nmeas * ncorr / wall, unit="moves/s", cores=1, kind="port", cpu=cpu_model(),
+                sample="1 chain x %d sweeps of %d moves from a thermalised configuration, %s" % (
+                    nmeas, ncorr, prob.label), wall_s=wall)
 
 
 def copy_probe(mib: int):
@@ -206,15 +301,16 @@ def copy_probe(mib: int):
                       "dispatches": 5}))
 
 
-def traffic_from_profiles(workload: str):
-    """HBM bytes per sweep-kernel dispatch measured with rocprofv3 --pmc for this
-    workload (profiles/sweep_kernel_traffic_<workload>.txt), if recorded."""
+def traffic_per_move(workload: str):
+    """HBM bytes per attempted move of the sweep kernel measured with rocprofv3
+    --pmc (FETCH_SIZE and WRITE_SIZE, scaled by a copy kernel) on this workload
+    at equilibrium (profiles/sweep_kernel_traffic_<workload>.txt), if recorded."""
     path = os.path.join(ROOT, "profiles", "sweep_kernel_traffic_%s.txt" % workload)
     if not os.path.exists(path):
         return None
     with open(path) as f:
         for line in f:
-            if line.startswith("traffic_bytes_per_dispatch:"):
+            if line.startswith("traffic_bytes_per_move:"):
                 return float(line.split(":", 1)[1])
     return None
 
@@ -222,12 +318,13 @@ def traffic_from_profiles(workload: str):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=400)
-    ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--config", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=10, help="minimum thermalisation sweeps")
+    ap.add_argument("--config", default="c4", choices=sorted(WORKLOADS))
     ap.add_argument("--chains", type=int, default=0)
     ap.add_argument("--ncorr", type=int, default=100)
     ap.add_argument("--seed", type=int, default=43890)
+    ap.add_argument("--therm-s", type=float, default=240.0, help="thermalisation time budget")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -241,32 +338,50 @@ def main():
               "(multi-GPU sharding is a design note, not built); run with --gpus 1",
               file=sys.stderr)
         sys.exit(2)
-    prob, tdm = make_problem(args.config, args.ncorr)
-    _, _, _, nchains_default, capacity = WORKLOADS[args.config]
-    nchains = args.chains or nchains_default
+    cfg = args.config
+    nchains = args.chains or WORKLOADS[cfg][3]
+    capacity = WORKLOADS[cfg][4]
     seed = args.seed
 
-    b = run_gpu(prob, tdm, nchains, capacity, args.warmup, args.steps, args.ncorr, seed)
+    b = run_gpu(cfg, nchains, capacity, args.warmup, args.steps, args.ncorr, seed, args.therm_s)
     value = b["moves"] / b["wall"]
     inv_err2_s = b["inv_err2"] / b["wall"]
     kernel_s = b["kernel_ms"] / 1e3
-    ach_gbs = b["alg_bytes"] / kernel_s / 1e9
+    launches = max(1, b["launches"])
+    # algorithmic bytes: state vectors and matrices counted per chain by the
+    # kernel, operator blocks (shared by all chains) once per launch
+    alg_bytes = b["alg_bytes"] + b["nops_bytes"] * launches
+    ach_gbs = alg_bytes / kernel_s / 1e9
     ach_tfs = b["alg_flops"] / kernel_s / 1e12
+    mfma = cfg == "c4"
+    tpm = traffic_per_move(cfg)
+    traffic = tpm * b["moves"] / launches if tpm else None
+    roof = {"bound": "mfma" if mfma else "hbm",
+            "achieved": ach_tfs if mfma else ach_gbs,
+            "peak": FP64_PEAK_TFS if mfma else HBM_PEAK_GBS,
+            "unit": "TFLOP/s" if mfma else "GB/s",
+            "frac": (ach_tfs / FP64_PEAK_TFS) if mfma else (ach_gbs / HBM_PEAK_GBS),
+            "traffic": traffic,
+            "kernel": "sweep_kernel", "launches": b["launches"], "avg_launch_ms": b["kernel_ms"] / launches,
+            "alg_flops_per_launch": b["alg_flops"] / launches, "alg_bytes_per_launch": alg_bytes / launches,
+            "fp64_tflops": ach_tfs, "fp64_frac": ach_tfs / FP64_PEAK_TFS,
+            "hbm_gbs": ach_gbs, "hbm_frac": ach_gbs / HBM_PEAK_GBS}
 
     plain = None
     if not args.no_plain:
-        psteps = max(32, args.steps // 4)
-        p = run_plain(prob, tdm, capacity, args.warmup, psteps, args.ncorr, seed)
+        p = run_plain(cfg, capacity, args.ncorr, seed + 1000003, b["configs"][0])
         plain = dict(moves_per_s=p["moves"] / p["wall"], gtau_inv_err2_per_s=p["inv_err2"] / p["wall"],
-                     sweeps=psteps, wall_s=p["wall"])
+                     sweeps=p["nmeas"], wall_s=p["wall"], mean_order=p["order"],
+                     error="binning plateau (AccumGtau=2): largest <stderr^2> over levels with >= 16 bins",
+                     plateau_curve=p["curve"])
 
     cpu = None
     if not args.no_cpu:
-        cprob, _ = make_problem(args.config, args.ncorr)
-        cpu = cpu_baseline(cprob, tdm, seed, nwarm=100000, nmeas=10000, ncorr=args.ncorr)
+        cpu = cpu_baseline(cfg, args.ncorr, seed, b["configs"])
 
+    st = b["status"]
     line = {
-        "metric": "MC moves/s (batched chains, 1 GPU)",
+        "metric": METRIC,
         "value": value,
         "unit": "moves/s",
         "n_gpus": 1,
@@ -277,20 +392,20 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (semicircular-bath impurity model)",
+        "data": "synthetic (semicircular-bath impurity model, seeded crystal field)",
         "config": {"workload": "%s: %s, beta=%g, %d batched chains, NCorr=%d, TaudiffMax=%g" % (
-            args.config, prob.label, prob.beta, nchains, args.ncorr, tdm),
-            "chains": nchains, "ncorr": args.ncorr, "mean_order_ops": b["order"]},
+            cfg, b["label"], b["beta"], nchains, args.ncorr, b["tdm"]),
+            "chains": nchains, "ncorr": args.ncorr,
+            "mean_order_ops_before": b["order_before"], "mean_order_ops_after": b["order_after"],
+            "thermalisation": [[s, round(o, 2), round(t, 1)] for s, o, t in b["traj"]],
+            "capacity": st["capacity"], "max_order_per_flavour": int(st["kfmax"].max()),
+            "chains_failed": int((st["status"] != 0).sum())},
         "gtau_inv_err2_per_s": inv_err2_s,
+        "mean_sign": b["mean_sign"],
         "plain_single_chain": plain,
         "batched_vs_plain_inv_err2_ratio": (inv_err2_s / plain["gtau_inv_err2_per_s"]) if plain else None,
         "batched_vs_plain_moves_ratio": (value / plain["moves_per_s"]) if plain else None,
-        "roofline": {"bound": "hbm", "achieved": ach_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": ach_gbs / HBM_PEAK_GBS, "traffic": traffic_from_profiles(args.config),
-                     "kernel": "sweep_kernel", "launches": b["launches"],
-                     "avg_launch_ms": b["kernel_ms"] / max(1, b["launches"]),
-                     "alg_bytes_per_launch": b["alg_bytes"] / max(1, b["launches"]),
-                     "fp64_tflops": ach_tfs, "fp64_frac": ach_tfs / FP64_PEAK_TFS},
+        "roofline": roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -59,13 +59,60 @@ typedef struct {
  *                value fixes the capacity at -capacity (chains that outgrow
  *                it stop with W2D_ERR_CAPACITY, the others carry on)
  * Screening (Uw) is not supported; offdiag, segment, statesampling, worm and
- * four-operator moves must be off. */
+ * four-operator moves must be off; so must the measurements this library does
+ * not produce (MeasGiw, MeasG2iw, MeasGSigmaiw, FourPnt, MeasSusz, MeasSsquare,
+ * MeasExpResDensityMatrix, MeasGtauDetRat, MeasurementTiming >= 0): W2D_ERR_ARG.
+ * AccumGtau 1 (mean) and 2 (binning) are supported. */
 int w2d_ctqmc_create(const char *params, int nbands, int nftau, const double *u_matrix,
                      const double *ftau_full, const double *muimp_full, int nchains,
                      uint32_t seed0, int capacity, int device, w2d_ctqmc **out);
 
-/* Replaces set_taudiffmax (CTQMC.F90:8180). */
+/* Replaces set_taudiffmax (CTQMC.F90:8180): a new window for every chain.
+ * After the chains have moved (the calibration of cthyb_solver.py:899-944)
+ * their configurations, random streams and inverses are kept. */
 int w2d_ctqmc_set_taudiffmax(w2d_ctqmc *h, double taudiff_max);
+
+/* Replaces init_counters (CTQMC.F90:847-899): move counters, the accepted-pair
+ * histogram AccPair and the index of the calibration list. */
+int w2d_ctqmc_init_counters(w2d_ctqmc *h);
+
+/* Replaces ctqmc_calibrate(list, phase1pairnum, phase2taugrid, phase2pairnum,
+ * nprogress) (CTQMC.F90:8088-8178) for every chain: Metropolis steps as in
+ * warm-up until, checked every min(1000, nprogress) steps, phase 1 (list != 0)
+ * has filled the chain's AccPairTau list with phase1pairnum accepted-pair
+ * distances, or phase 2 (list = 0) has phase2pairnum accepted pairs of every
+ * flavour in AccPair over phase2taugrid bins of [0, taudiff_max].  Each chain
+ * stops on its own condition; the call returns when all have. */
+int w2d_ctqmc_calibrate(w2d_ctqmc *h, int list, int phase1pairnum, int phase2taugrid,
+                        int phase2pairnum, int nprogress);
+
+/* Accepted-pair statistics of one chain (AccPairTau, AccPair; CTQMC.F90:66-70):
+ * accpairtau[*napt] the distances recorded by calibration phase 1, accpair
+ * [2N][*naccpair] the accepted insertions and removals per flavour and
+ * distance bin since the last init_counters (counts).  Pointers may be NULL. */
+int w2d_ctqmc_get_pair_stats(w2d_ctqmc *h, int chain, double *accpairtau, int *napt, double *accpair,
+                             int *naccpair);
+
+/* MeasOuterHistograms (CTQMC.F90:3285-3339) and AccPair summed over groups of
+ * chains as w2d_ctqmc_get_group_sums does, raw (before the division by the
+ * measurement count and visits of CTQMC.F90:8757-8764):
+ *   outer_state [ngroups][nstates]  OuterStateHisto (visits of each outer state)
+ *   sign_sst    [ngroups][nsst]     SignHistoSuperstates (sum of cfgsign)
+ *   sign_state  [ngroups][nstates]  SignHistoStates
+ *   contrib_sst [ngroups][nsst]     TraceContribSuperstates
+ *   contrib_state [ngroups][nstates] TraceContribStates
+ *   accpair     [ngroups][2N][naccpair capacity]  (w2d_ctqmc_dims' ntau or 1024,
+ *               whichever is larger; the first naccpair bins are used)
+ * Any pointer may be NULL. */
+int w2d_ctqmc_get_group_histograms(w2d_ctqmc *h, int ngroups, double *outer_state, double *sign_sst,
+                                   double *sign_state, double *contrib_sst, double *contrib_state,
+                                   double *accpair);
+
+/* Step-resolved measurements of one chain (Gtau_mean_step, Gtau_mid_step,
+ * sign_step; CTQMC.F90:2394-2402), raw, [*nmstep][2N] and [*nmstep], where
+ * *nmstep = Nmeas when any of them is enabled, else 0.  Pointers may be NULL. */
+int w2d_ctqmc_get_steps(w2d_ctqmc *h, int chain, int *nmstep, double *gtau_mean_step,
+                        double *gtau_mid_step, double *sign_step);
 
 /* Replaces ctqmc_warmup (CTQMC.F90:8188): nwarmups Metropolis steps per chain.
  * Returns W2D_ERR_PARTIAL if some (not all) chains stopped on an error. */
@@ -107,6 +154,13 @@ int w2d_ctqmc_get_accumulators(w2d_ctqmc *h, double *gtau, double *gleg, double 
  * normalisation CTQMC.F90:8663-8771 and, over groups, its jackknife error. */
 int w2d_ctqmc_get_group_sums(w2d_ctqmc *h, int ngroups, double *gtau, double *gleg, double *dm_blocks,
                              double *histo, double *outer, double *scalars, int32_t *nused);
+
+/* AccumGtau = 2 (autocorracc, Accumulators.F90:283-362): binning analysis of
+ * one chain's raw G(tau) bins, blocks[l][2N][ntau] = variance of the means of
+ * bins of 2^l measurements (gtau_blocks, CTQMC.F90:8733-8739); *nlevels =
+ * ilog2(Nmeas) (blocks may be NULL to query it).  W2D_ERR_STATE unless the
+ * handle was created with AccumGtau = 2. */
+int w2d_ctqmc_get_gtau_blocks(w2d_ctqmc *h, int chain, int *nlevels, double *blocks);
 
 /* Eigenbasis of the local Hamiltonian built by create: per-superstate sizes and
  * offsets, eigenvalues (EigenstatesEnergies), occupation-basis state of each
@@ -152,6 +206,16 @@ int w2d_ctqmc_last_kernel_ms(const w2d_ctqmc *h, double *ms, int64_t *launches);
  * MatrixUpdate.F90:410-482).  Any pointer may be NULL.  No reference
  * counterpart (the reference stops the process, one rank per chain). */
 int w2d_ctqmc_chain_status(const w2d_ctqmc *h, int32_t *status, int32_t *kfmax, int *capacity);
+
+/* Bytes of the eigenbasis operator blocks (Psis_EB, States.F90:40-42) the
+ * trace reads; shared by all chains, so the roofline charges them once per
+ * launch. */
+int w2d_ctqmc_op_table_bytes(const w2d_ctqmc *h, int64_t *bytes);
+
+/* Operators in the configuration (2 x expansion order) and outer superstate of
+ * every chain (get_mc_config_scalars, CTQMC.F90:9358, for all chains at once);
+ * outer_sst may be NULL. */
+int w2d_ctqmc_get_orders(const w2d_ctqmc *h, int32_t *nop, int32_t *outer_sst);
 
 /* Abort word of the handle: flag != 0 makes running and later sweeps stop at
  * the next measurement boundary (every 128 moves in warm-up), keeping the data

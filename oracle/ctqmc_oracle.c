@@ -157,6 +157,10 @@ typedef struct {
     double *sign_hist_sst, *sign_hist_states, *tc_sst, *tc_states;
     double meas_sign;
     long cnt_meas;
+    /* AccumGtau = 2: binning accumulator of G(tau) (autocorracc) */
+    int accum, nlev;
+    long acc_n;
+    double *gmeas, *acc_spill, *acc_mean, *acc_sq;   /* [ncomp], [nlev][ncomp] x 3 */
     /* scratch */
     double *ket, *tmp;
     double *kets_b2, *bras_b2, *klog_b2, *blog_b2;
@@ -1053,6 +1057,71 @@ static int step_shift(ochain_t *ch)
     return 0;
 }
 
+/* ------------------------- binning accumulator ------------------------------ */
+/* autocorracc_add (Accumulators.F90:298-340): level 0 takes the new value by
+ * the Welford update of mean and squared deviations; whenever the low bits of
+ * the count are ones, the finished bin of a level (the mean of its two halves)
+ * rolls over into the next level.  Arrays [nlev][ncomp]; `n` values so far. */
+static void oacc_add(long ncomp, int nlev, long n, const double *val, double *spill, double *mean,
+                     double *sq)
+{
+    for (long i = 0; i < ncomp; ++i) {
+        double v = val[i];
+        spill[i] += v;
+        double d1 = v - mean[i];
+        mean[i] += d1 / (double)(n + 1);
+        sq[i] += d1 * (v - mean[i]);
+    }
+    int lvl = 0;
+    while ((n & 1) != 0) {
+        if (lvl == nlev - 1) break;
+        n /= 2;
+        ++lvl;
+        double *sp0 = spill + (long)(lvl - 1) * ncomp, *sp = spill + (long)lvl * ncomp;
+        double *mn = mean + (long)lvl * ncomp, *s2 = sq + (long)lvl * ncomp;
+        for (long i = 0; i < ncomp; ++i) {
+            double v = sp0[i] / 2.0;
+            sp0[i] = 0.0;
+            sp[i] += v;
+            double d1 = v - mn[i];
+            mn[i] += d1 / (double)(n + 1);
+            s2[i] += d1 * (v - mn[i]);
+        }
+    }
+}
+
+/* autocorracc_blocks (Accumulators.F90:342-362): variance of the bin means of
+ * level l, sqdiff / (n_l - 1) with n_l = n >> l */
+static void oacc_blocks(long ncomp, int nlev, long n, const double *sq, double *out)
+{
+    for (int l = 0; l < nlev; ++l) {
+        for (long i = 0; i < ncomp; ++i)
+            out[(long)l * ncomp + i] = sq[(long)l * ncomp + i] / (n > 1 ? (double)(n - 1) : 0.0);
+        n /= 2;
+    }
+}
+
+/* the accumulator on a given sequence (data [n][ncomp]): mean [ncomp] and
+ * blocks [nlev][ncomp]; pinned by the closed form of the blocking variances */
+void oracle_autocorr(long ncomp, int nlev, long n, const double *data, double *mean_out, double *blocks_out)
+{
+    double *sp = calloc((size_t)nlev * ncomp, sizeof(double));
+    double *mn = calloc((size_t)nlev * ncomp, sizeof(double));
+    double *s2 = calloc((size_t)nlev * ncomp, sizeof(double));
+    for (long j = 0; j < n; ++j) oacc_add(ncomp, nlev, j, data + j * ncomp, sp, mn, s2);
+    memcpy(mean_out, mn, sizeof(double) * (size_t)ncomp);
+    oacc_blocks(ncomp, nlev, n, s2, blocks_out);
+    free(sp), free(mn), free(s2);
+}
+
+/* ilog2 (Accumulators.F90:89-101): floor(log2(v)), -1 for v < 1 */
+static int ilog2_l(long v)
+{
+    int r = -1;
+    while (v > 0) ++r, v /= 2;
+    return r;
+}
+
 /* ------------------------- measurements --------------------------------- */
 /* MeasGtauRem (CTQMC.F90:2252-2440) */
 static void meas_gtau(ochain_t *ch)
@@ -1098,10 +1167,17 @@ static void meas_gtau(ochain_t *ch)
                 pl = plp1;
             }
             if (idt > ch->ntau) idt = ch->ntau;
-            ch->gtau_buf[(long)f * ch->ntau + idt - 1] += M;
+            (ch->accum == 2 ? ch->gmeas : ch->gtau_buf)[(long)f * ch->ntau + idt - 1] += M;
         }
     }
-    (void)nf;
+    /* gtau_accum%add() (CTQMC.F90:2439) */
+    if (ch->accum == 2) {
+        long nc = (long)nf * ch->ntau;
+        for (long i = 0; i < nc; ++i) ch->gtau_buf[i] += ch->gmeas[i];
+        oacc_add(nc, ch->nlev, ch->acc_n, ch->gmeas, ch->acc_spill, ch->acc_mean, ch->acc_sq);
+        ch->acc_n++;
+        memset(ch->gmeas, 0, sizeof(double) * (size_t)nc);
+    }
 }
 
 /* MeasDensityMatrix_beta_half (CTQMC.F90:3088-3151) */
@@ -1218,6 +1294,10 @@ void *oracle_create(int nb, int nsst, int nmax, const int *sizes, const int *off
     ch->p_shift = dparams[0], ch->p_glob = dparams[1], ch->p_outer = dparams[2], ch->p_4op = dparams[3];
     ch->ntau = (int)iparams[1], ch->nleg = (int)iparams[2], ch->maxhisto = (int)iparams[3];
     ch->has_szt = (int)iparams[4], ch->meas_dm = (int)iparams[5];
+    /* iparams[6] AccumGtau, iparams[7] Nmeas (levels = ilog2(Nmeas), CTQMC.F90:497-500) */
+    ch->accum = (int)iparams[6];
+    ch->nlev = ilog2_l(iparams[7]);
+    if (ch->nlev < 1) ch->nlev = 1;
     ch->cap = 1024;
     ch->op = malloc(sizeof(oop_t) * (size_t)ch->cap);
     ch->ket = malloc(sizeof(double) * (size_t)nmax);
@@ -1232,6 +1312,10 @@ void *oracle_create(int nb, int nsst, int nmax, const int *sizes, const int *off
     for (int f = 0; f < nf; ++f) ch->minv[f] = malloc(sizeof(double)), ch->k[f] = 0;
     ch->gtau_buf = calloc((size_t)nf * ch->ntau, sizeof(double));
     ch->gleg = calloc((size_t)nf * ch->nleg, sizeof(double));
+    ch->gmeas = calloc((size_t)nf * ch->ntau, sizeof(double));
+    ch->acc_spill = calloc((size_t)nf * ch->ntau * ch->nlev, sizeof(double));
+    ch->acc_mean = calloc((size_t)nf * ch->ntau * ch->nlev, sizeof(double));
+    ch->acc_sq = calloc((size_t)nf * ch->ntau * ch->nlev, sizeof(double));
     ch->dm = calloc((size_t)ch->nstates * ch->nstates, sizeof(double));
     ch->histo = calloc((size_t)nf * (ch->maxhisto + 1), sizeof(double));
     ch->outer_hist = calloc((size_t)nsst, sizeof(double));
@@ -1275,6 +1359,7 @@ void oracle_destroy(void *h)
     free(ch->op), free(ch->ket), free(ch->tmp), free(ch->kets_b2), free(ch->bras_b2);
     free(ch->klog_b2), free(ch->blog_b2), free(ch->parttrace), free(ch->tpart);
     free(ch->gtau_buf), free(ch->gleg), free(ch->dm), free(ch->histo), free(ch->outer_hist);
+    free(ch->gmeas), free(ch->acc_spill), free(ch->acc_mean), free(ch->acc_sq);
     free(ch->sign_hist_sst), free(ch->tc_sst), free(ch->outer_state_hist);
     free(ch->sign_hist_states), free(ch->tc_states);
     free(ch);
@@ -1321,6 +1406,14 @@ void oracle_measure(void *h, long nmeas, long ncorr)
         if (ch->meas_dm) meas_dm(ch);
         meas_gtau(ch);
     }
+}
+
+/* binning blocks of G(tau) (AccumGtau = 2): [nlev][nf*ntau]; returns nlev */
+int oracle_gtau_blocks(void *h, double *out)
+{
+    ochain_t *ch = h;
+    if (out) oacc_blocks((long)NF(ch) * ch->ntau, ch->nlev, ch->acc_n, ch->acc_sq, out);
+    return ch->nlev;
 }
 
 /* raw accumulators and counters (before normalisation) */

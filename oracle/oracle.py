@@ -54,11 +54,25 @@ def lib():
         _lib.oracle_get_minv.restype = C.c_int
         _lib.oracle_set_config.argtypes = [P, C.c_int, P, P, P, P, C.c_int, P]
         _lib.oracle_rng_stream.argtypes = [C.c_uint, C.c_long, P, P, P, P]
+        _lib.oracle_gtau_blocks.argtypes = [P, P]
+        _lib.oracle_gtau_blocks.restype = C.c_int
+        _lib.oracle_autocorr.argtypes = [C.c_long, C.c_int, C.c_long, P, P, P]
     return _lib
 
 
 def _p(a):
     return a.ctypes.data_as(C.c_void_p)
+
+
+def autocorr(data: np.ndarray, nlev: int):
+    """The binning accumulator (AccumGtau = 2) on a sequence data[n][ncomp]:
+    (mean, blocks[nlev][ncomp])."""
+    data = np.ascontiguousarray(data, dtype=float)
+    n, nc = data.shape
+    mean = np.zeros(nc)
+    blocks = np.zeros((nlev, nc))
+    lib().oracle_autocorr(nc, int(nlev), n, _p(data), _p(mean), _p(blocks))
+    return mean, blocks
 
 
 def rng_stream(seed: int, kinds, lo=None, hi=None) -> np.ndarray:
@@ -99,7 +113,9 @@ class OracleChain:
         self._keep["iparams"] = np.array([int(qmc["NCorr"]), int(qmc["Ntau"]), int(qmc["NLegMax"]),
                                           int(qmc["MaxHisto"]),
                                           int("Szt" in quantum_numbers),
-                                          int(qmc["MeasDensityMatrix"])], dtype=np.int64)
+                                          int(qmc["MeasDensityMatrix"]),
+                                          int(qmc.get("AccumGtau", 1)),
+                                          int(qmc["Nmeas"])], dtype=np.int64)
         for key in ("opoff", "evoff"):
             self._keep[key] = np.ascontiguousarray(self._keep[key], dtype=np.int64)
         k = self._keep
@@ -165,6 +181,13 @@ class OracleChain:
         lib().oracle_set_config(self.h, len(taus), _p(taus), _p(orbs), _p(spins), _p(cas),
                                 int(outer_sst), _p(w))
         return dict(trace_log=w[0], trace_sign=w[1], det_log=w[2], det_sign=w[3], cfgsign=w[4])
+
+    def gtau_blocks(self) -> np.ndarray:
+        """AccumGtau = 2: variances of the level-l bin means, [nlev][flavour][ntau]."""
+        nlev = lib().oracle_gtau_blocks(self.h, None)
+        out = np.zeros(nlev * 2 * self.nb * self.ntau)
+        lib().oracle_gtau_blocks(self.h, _p(out))
+        return out.reshape(nlev, 2 * self.nb, self.ntau)
 
     def raw(self) -> dict:
         nf = 2 * self.nb

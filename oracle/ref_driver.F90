@@ -10,6 +10,9 @@
 !   run   : ctqmc_warmup(NWARM) + ctqmc_measure(1,1) and dump the observables
 !   eval  : set_mc_config for each configuration in <dir>/configs.bin and dump the
 !           trace / determinant / inverse hybridisation matrices
+!   runcfg: set_mc_config with the first configuration of <dir>/configs.bin (a
+!           thermalised state, as ReuseMCConfig does), then as `run`; also
+!           writes <dir>/timing.txt (time_sim of the sampling and the final order)
 !
 ! run.txt layout (free format, one item per line):
 !   mode, nb, nft, seed, nwarm, nmoves, taudiffmax
@@ -81,6 +84,14 @@ program ref_driver
       call write_results(trim(dir))
    case ('eval')
       call run_eval(trim(dir))
+   case ('runcfg')
+      call start_from_config(trim(dir))
+      call ctqmc_warmup(nwarm)
+      call ctqmc_measure(1, 1)
+      call write_results(trim(dir))
+      open(80, file=trim(dir)//'/timing.txt', action='write', status='replace')
+      write(80, '(2ES25.17, I10)') time_warmup, time_sim, DTrace%NOper
+      close(80)
    case default
       stop 'unknown mode'
    end select
@@ -226,6 +237,23 @@ contains
       close(60)
       close(61)
    end subroutine run_eval
+
+   subroutine start_from_config(d)
+      character(*), intent(in) :: d
+      integer(c_int32_t) :: ncfg, n, osst
+      real(c_double), allocatable :: taus(:)
+      integer(c_int32_t), allocatable :: o(:), s(:), c(:)
+      integer, allocatable :: hyb(:)
+      open(60, file=d//'/configs.bin', access='stream', form='unformatted', status='old', action='read')
+      read(60) ncfg
+      read(60) n, osst
+      allocate(taus(n), o(n), s(n), c(n), hyb(n))
+      read(60) taus
+      read(60) o, s, c
+      close(60)
+      hyb = -1
+      call set_mc_config(int(n), taus, int(o), int(s), int(c), hyb, int(osst), 1)
+   end subroutine start_from_config
 
    subroutine write_results(d)
       character(*), intent(in) :: d

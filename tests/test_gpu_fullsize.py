@@ -295,3 +295,27 @@ def test_batched_chains_match_reference_runs():
         dev = np.abs(a - b) / sig
         print(k, "largest deviation in sigma", float(dev.max()), flush=True)
         assert np.mean(dev <= 3) >= 0.97 and dev.max() <= 4.5, (k, float(dev.max()))
+
+
+def test_binning_accumulator_matches_oracle():
+    """AccumGtau = 2: the binning levels of G(tau) (gtau_blocks) and the mean
+    agree with the oracle's autocorracc on the same chain."""
+    prob = P.kanamori_2orb(beta=10.0, TaudiffMax=3.0, Nmeas=300, NCorr=20, Ntau=40, NLegMax=10,
+                           AccumGtau=2)
+    seed = 61
+    gpu = CtqmcGpu.from_problem(prob, nchains=2, seed=seed)
+    gpu.set_taudiffmax(3.0)
+    gpu.warmup(2000)
+    gpu.measure(300)
+    res = gpu.results()
+    ora = OracleChain.from_problem(prob, seed)
+    ora.set_taudiffmax(3.0)
+    ora.warmup(2000)
+    ora.measure(300, 20)
+    a, b = gpu.gtau_blocks(0), ora.gtau_blocks()
+    assert a.shape == b.shape == (8, 4, 40)
+    fin = np.isfinite(b)
+    assert np.array_equal(fin, np.isfinite(a))
+    assert np.allclose(a[fin], b[fin], rtol=1e-9, atol=1e-13)
+    assert np.allclose(res["gtau"][0], ora.results()["gtau"], rtol=1e-9, atol=1e-12)
+    gpu.close()

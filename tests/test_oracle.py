@@ -291,3 +291,36 @@ def test_full_run_matches_exact_diagonalisation():
         m = np.mean(vals)
         se = np.std(vals, ddof=1) / np.sqrt(len(vals))
         assert abs(m - exact) < 4 * se + 2e-3, (m, se, exact)
+
+
+def test_binning_accumulator_closed_form():
+    """autocorracc (Accumulators.F90:283-362) on the sequence 0, 1, ..., 2^n - 1
+    (the reference's own test program, Accumulators.F90:366-410): the bin
+    means of level l form an arithmetic sequence of step 2^l and length
+    N = 2^(n-l), whose variance is 4^l N (N + 1) / 12; the mean is (2^n - 1)/2."""
+    import oracle as O
+    n = 12
+    data = np.arange(2 ** n, dtype=float)[:, None]
+    mean, blocks = O.autocorr(data, n)
+    assert abs(mean[0] - (2 ** n - 1) / 2) < 1e-9
+    for lvl in range(n):
+        N = 2 ** (n - lvl)
+        exact = 4.0 ** lvl * N * (N + 1) / 12.0
+        assert abs(blocks[lvl, 0] - exact) <= 1e-9 * exact, (lvl, blocks[lvl, 0], exact)
+
+
+def test_binning_accumulator_matches_bin_means():
+    """On random data of a length that is not a power of two, level l holds the
+    sample variance of the means of the complete 2^l-bins."""
+    import oracle as O
+    rng = np.random.default_rng(3)
+    n, nc, nlev = 1000, 3, 6
+    data = rng.normal(size=(n, nc)) + np.arange(nc)
+    mean, blocks = O.autocorr(data, nlev)
+    assert np.allclose(mean, data.mean(0), rtol=0, atol=1e-12)
+    for lvl in range(nlev):
+        w = 2 ** lvl
+        nb = n // w
+        bins = data[:nb * w].reshape(nb, w, nc).mean(1)
+        ref = bins.var(0, ddof=1) * (nb - 1) / ((n >> lvl) - 1)
+        assert np.allclose(blocks[lvl], ref, rtol=1e-10), lvl

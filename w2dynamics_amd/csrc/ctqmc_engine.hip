@@ -42,6 +42,7 @@ struct w2d_ctqmc {
     int aborted = 0;           // of the last warmup / measure (CTQMC.F90:8648-8658)
     int regrows = 0;           // capacity doublings so far
     bool grow = true;          // capacity doubles on demand (create's capacity >= 0)
+    int3 calib{0, 1, 0};       // calibration mode of the sweeps: phase, check period, pairs per flavour
     ~w2d_ctqmc();
 };
 
@@ -74,6 +75,9 @@ int upload(w2d_ctqmc *h, const std::vector<T> &v, const T **out)
 long align256(long x) { return (x + 255) / 256 * 256; }
 
 constexpr int KCAP_MAX = 4096;   // pairs per flavour a chain may grow to
+constexpr int ACCPAIR_BINS_MAX = 1024;   // accepted-pair histogram bins (NAccPair) at most, beyond Ntau
+constexpr int APT_CAP = 65536;           // accepted-pair distances a calibration list holds at most
+
 
 // abort words of the live handles, raised together by the signal handler
 constexpr int MAX_TRAPPED = 64;
@@ -176,6 +180,23 @@ int compute_layout(const w2d_ctqmc *h, int nbound, int kc, w2d::ChainLayout &L)
     L.dm = take(h->dm.dmsize * 8);
     L.histo = take((long)nf * (h->maxhisto + 1) * 8);
     L.outer = take((long)h->model.nsst * 8);
+    const bool bins = h->dm.accum == 2;
+    L.gbuf = take(bins ? (long)nf * h->ntau * 8 : 8);
+    L.gacc = take(bins ? 3L * h->dm.nlev * nf * h->ntau * 8 : 8);
+    const long nst = h->model.nstates, nss = h->model.nsst;
+    L.ohst = take(nst * 8);
+    L.shss = take(nss * 8);
+    L.shst = take(nst * 8);
+    L.tcss = take(nss * 8);
+    L.tcst = take(nst * 8);
+    L.naccpair_cap = std::max(h->ntau, ACCPAIR_BINS_MAX);
+    L.accpair = take((long)nf * L.naccpair_cap * 8);
+    const long nms = h->dm.nmstep;
+    L.gmstep = take(h->dm.step_mean ? nms * nf * 8 : 8);
+    L.gdstep = take(h->dm.step_mid ? nms * nf * 8 : 8);
+    L.sstep = take(h->dm.step_sign ? nms * 8 : 8);
+    L.apt_cap = APT_CAP;
+    L.aptau = take((long)L.apt_cap * 8);
     L.stride = o;
     // dynamic LDS
     const w2d::LocalModel &m = h->model;
@@ -269,14 +290,13 @@ int layout_and_alloc(w2d_ctqmc *h, int nbound)
     return W2D_OK;
 }
 
-// Double the pairs-per-flavour capacity of every chain, keeping each chain's
-// state (a chain halted for room continues where it stopped).
-int regrow(w2d_ctqmc *h)
+// Move every chain into a layout of capacity kc and nbound window boundaries,
+// keeping each chain's state (configuration, random stream, inverses,
+// accumulators; boundary caches when the window count stays).
+int relayout(w2d_ctqmc *h, int kc, int nbound)
 {
-    const int kc = h->kcap * 2;
-    if (!h->grow || kc > KCAP_MAX) return W2D_ERR_CAPACITY;
     w2d::ChainLayout B;
-    int st = compute_layout(h, h->L.nbound, kc, B);
+    int st = compute_layout(h, nbound, kc, B);
     if (st) return st;
     char *nc = nullptr;
     HIPCHK(hipStreamSynchronize(h->stream));
@@ -291,8 +311,19 @@ int regrow(w2d_ctqmc *h)
     h->chains = nc;
     h->L = B;
     h->kcap = kc;
-    h->regrows++;
+    h->nwin_alloc = nbound;
     return W2D_OK;
+}
+
+// Double the pairs-per-flavour capacity of every chain (a chain halted for
+// room continues where it stopped).
+int regrow(w2d_ctqmc *h)
+{
+    const int kc = h->kcap * 2;
+    if (!h->grow || kc > KCAP_MAX) return W2D_ERR_CAPACITY;
+    const int st = relayout(h, kc, h->L.nbound);
+    if (!st) h->regrows++;
+    return st;
 }
 
 // allow the dynamic LDS of this handle (above 64 KB on devices that have it)
@@ -318,7 +349,7 @@ void launch_sweep(w2d_ctqmc *h, long long target, long long ncorr, int only, w2d
 {
     allow_lds(w2d::sweep_kernel<NM>, h);
     hipLaunchKernelGGL(w2d::sweep_kernel<NM>, dim3(h->nchains), dim3(w2d::WAVE), (unsigned)h->L.lds_bytes, h->stream, h->dm, h->L,
-                       h->chains, target, ncorr, only, rec, rec_base, (const int *)h->abort_dev);
+                       h->chains, target, ncorr, only, rec, rec_base, (const int *)h->abort_dev, h->calib);
 }
 
 template <int NM>
@@ -376,6 +407,11 @@ int init_chains(w2d_ctqmc *h)
         hd.trace_sign = 1.0;
         hd.cfgsign = -1.0;
         hd.mti = 624;
+        // AccPair over NGtau bins of [0, beta] (CTQMC.F90:775-777)
+        hd.naccpair = h->ntau;
+        hd.accpair_max = h->beta;
+        hd.apt_index = 1;
+        hd.apt_n = 0;
         init_window(hd, h->taudiff_param, h->beta);
         seed_mt(mt.data() + 624L * c, h->seed0 + (uint32_t)c);
     }
@@ -400,6 +436,16 @@ void reset_counters(ChainHdr &x)
     x.alg_flops = x.alg_bytes = 0.0;
     x.pos = 0;
     x.halt = w2d::HALT_NONE;
+    x.apt_index = 1;
+}
+
+// AccPair = 0 on every chain (init_counters, CTQMC.F90:851)
+int clear_accpair(w2d_ctqmc *h)
+{
+    const w2d::ChainLayout &L = h->L;
+    HIPCHK(hipMemset2DAsync(h->chains + L.accpair, (size_t)L.stride, 0,
+                            (size_t)2 * h->model.nbands * L.naccpair_cap * 8, (size_t)h->nchains, h->stream));
+    return W2D_OK;
 }
 
 int map_err(int e)
@@ -500,6 +546,48 @@ int run_moves(w2d_ctqmc *h, long long nmoves, long long ncorr)
     return batch_status(hd);
 }
 
+// Device work area of the group sums: [ngroups][nel] partial sums, then
+// [ngroups][14] scalars, then [ngroups] chain counts.
+size_t group_sums_bytes(const w2d_ctqmc *h, int ngroups)
+{
+    const long nel = (h->L.stride - h->L.gtau) / 8;
+    return (size_t)ngroups * ((size_t)nel * 8 + 14 * 8 + 8);
+}
+
+// Group sums into a caller-provided device work area; the results go straight
+// to the host arrays (strided copies), so nothing is allocated here.
+int group_sums_into(w2d_ctqmc *h, int ngroups, void *work, double *gtau, double *gleg, double *dm_blocks,
+                    double *histo, double *outer, double *scalars, int32_t *nused)
+{
+    const w2d::ChainLayout &L = h->L;
+    const int nf = 2 * h->model.nbands, nc = h->nchains;
+    const long nel = (L.stride - L.gtau) / 8;
+    double *part = static_cast<double *>(work);
+    double *scal = part + (long)ngroups * nel;
+    int *used = reinterpret_cast<int *>(scal + (long)ngroups * 14);
+    const ChainHdr *hdrs = reinterpret_cast<const ChainHdr *>(h->chains + L.hdrs);
+    hipLaunchKernelGGL(w2d::group_sum_kernel, dim3((unsigned)((nel + 255) / 256), (unsigned)ngroups), dim3(256), 0,
+                       h->stream, (const char *)h->chains, L.stride, L.gtau, nel, nc, ngroups, hdrs, part);
+    hipLaunchKernelGGL(w2d::group_scalars_kernel, dim3((unsigned)((ngroups + 63) / 64)), dim3(64), 0, h->stream, hdrs,
+                       nc, ngroups, scal, used);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->stream));
+    auto put = [&](double *dst, long off, long n) -> int {
+        if (!dst || n <= 0) return W2D_OK;
+        HIPCHK(hipMemcpy2D(dst, (size_t)n * 8, part + (off - L.gtau) / 8, (size_t)nel * 8, (size_t)n * 8,
+                           (size_t)ngroups, hipMemcpyDeviceToHost));
+        return W2D_OK;
+    };
+    int st;
+    if ((st = put(gtau, L.gtau, (long)nf * h->ntau)) || (st = put(gleg, L.gleg, (long)nf * h->nleg)) ||
+        (st = put(dm_blocks, L.dm, h->dm.dmsize)) || (st = put(histo, L.histo, (long)nf * (h->maxhisto + 1))) ||
+        (st = put(outer, L.outer, h->model.nsst)))
+        return st;
+    if (scalars) HIPCHK(hipMemcpy(scalars, scal, (size_t)ngroups * 14 * 8, hipMemcpyDeviceToHost));
+    if (nused) HIPCHK(hipMemcpy(nused, used, (size_t)ngroups * 4, hipMemcpyDeviceToHost));
+    return W2D_OK;
+}
+
 }  // namespace
 
 // every resource the handle holds (also on the error paths of create)
@@ -537,6 +625,21 @@ int w2d_ctqmc_create(const char *params, int nbands, int nftau, const double *u_
         P.integer("NSymMove") > 0 || P.integer("Eigenbasis") == 0)
         return W2D_ERR_ARG;
     if (P.integer("Nd") != nbands) return W2D_ERR_ARG;
+    // measurements this library does not produce: refused rather than left
+    // unmeasured (the caller would read arrays that were never filled)
+    {
+        const char *off_keys[] = {"MeasGiw", "MeasG2iw", "MeasGSigmaiw", "FourPnt", "MeasSusz", "MeasSsquare",
+                                  "MeasExpResDensityMatrix", "MeasGtauDetRat", "WormMeasGiw", "WormMeasGtau"};
+        for (const char *k : off_keys)
+            if (P.has(k) && P.integer(k) != 0) {
+                std::fprintf(stderr, "w2d_ctqmc: %s = %lld is not supported (only 0)\n", k, P.integer(k));
+                return W2D_ERR_ARG;
+            }
+        if (P.has("MeasurementTiming") && P.integer("MeasurementTiming") >= 0) {
+            std::fprintf(stderr, "w2d_ctqmc: MeasurementTiming >= 0 is not supported\n");
+            return W2D_ERR_ARG;
+        }
+    }
     auto h = std::make_unique<w2d_ctqmc>();
     h->params = params;
     h->nchains = nchains;
@@ -633,6 +736,20 @@ int w2d_ctqmc_create(const char *params, int nbands, int nftau, const double *u_
     d.maxhisto = h->maxhisto;
     d.has_szt = P.str("QuantumNumbers").find("Szt") != std::string::npos;
     d.meas_dm = P.integer("MeasDensityMatrix") != 0;
+    // AccumGtau: 1 meanacc, 2 autocorracc with ilog2(Nmeas) levels (CTQMC.F90:491-503)
+    d.accum = P.has("AccumGtau") ? (int)P.integer("AccumGtau") : 1;
+    if (d.accum != 1 && d.accum != 2) return W2D_ERR_ARG;
+    // step-resolved G(tau) and sign over the Nmeas measurements (CTQMC.F90:510-512)
+    d.step_mean = P.has("Gtau_mean_step") && P.integer("Gtau_mean_step") != 0;
+    d.step_mid = P.has("Gtau_mid_step") && P.integer("Gtau_mid_step") != 0;
+    d.step_sign = P.has("sign_step") && P.integer("sign_step") != 0;
+    d.nmstep = (d.step_mean || d.step_mid || d.step_sign) ? (int)std::max<long long>(P.integer("Nmeas"), 1) : 0;
+    {
+        long long nm = P.integer("Nmeas");
+        int lv = -1;
+        while (nm > 0) ++lv, nm /= 2;
+        d.nlev = std::max(lv, 1);
+    }
     // NSlide = max(2*NBands, floor(NCorr/num_windows)) with the window of the
     // parameter file (CTQMC.F90:8069)
     int nw0 = nbound_for(h->taudiff_param, h->beta) - 2;
@@ -650,13 +767,12 @@ int w2d_ctqmc_set_taudiffmax(w2d_ctqmc *h, double tdm)
     int nb = nbound_for(tdm, h->beta);
     if (nb != h->nwin_alloc) {
         // the window count sizes the boundary caches and the LDS operator
-        // lists: re-create the chains' memory.  Only before the chains have
-        // moved (as the reference's driver calls it), since their initial
-        // state is re-derived from the seeds.
-        if (h->moved) return W2D_ERR_STATE;
-        int st = layout_and_alloc(h, nb);
+        // lists: before the chains have moved, re-create their memory (their
+        // initial state follows from the seeds); after (the calibration of
+        // cthyb_solver.py:899-944), move them into the new layout
+        int st = h->moved ? relayout(h, h->kcap, nb) : layout_and_alloc(h, nb);
         if (st) return st;
-        if ((st = init_chains(h))) return st;
+        if (!h->moved && (st = init_chains(h))) return st;
     }
     std::vector<ChainHdr> hd;
     int st = read_headers(h, hd);
@@ -671,8 +787,14 @@ int w2d_ctqmc_warmup(w2d_ctqmc *h, int64_t nwarmups)
     std::vector<ChainHdr> hd;
     int st = read_headers(h, hd);
     if (st) return st;
-    for (auto &x : hd) reset_counters(x);
-    if ((st = write_headers(h, hd))) return st;
+    // ctqmc_warmup: AccPair over NGtau bins of [0, beta], then init_counters
+    // (CTQMC.F90:8199-8206)
+    for (auto &x : hd) {
+        reset_counters(x);
+        x.naccpair = h->ntau;
+        x.accpair_max = h->beta;
+    }
+    if ((st = write_headers(h, hd)) || (st = clear_accpair(h))) return st;
     return run_moves(h, nwarmups, 0);
 }
 
@@ -941,60 +1063,159 @@ int w2d_ctqmc_get_group_sums(w2d_ctqmc *h, int ngroups, double *gtau, double *gl
 {
     if (!h || ngroups < 1 || ngroups > h->nchains) return W2D_ERR_ARG;
     if (!h->measured) return W2D_ERR_STATE;
-    const w2d::ChainLayout &L = h->L;
-    const int nf = 2 * h->model.nbands, nc = h->nchains;
+    void *work = nullptr;
+    HIPCHK(hipMalloc(&work, group_sums_bytes(h, ngroups)));
+    const int st = group_sums_into(h, ngroups, work, gtau, gleg, dm_blocks, histo, outer, scalars, nused);
+    (void)hipFree(work);
+    return st;
+}
+
+int w2d_ctqmc_get_gtau_blocks(w2d_ctqmc *h, int chain, int *nlevels, double *blocks)
+{
+    if (!h || chain < 0 || chain >= h->nchains || !nlevels) return W2D_ERR_ARG;
+    if (h->dm.accum != 2) return W2D_ERR_STATE;
+    *nlevels = h->dm.nlev;
+    if (!blocks) return W2D_OK;
+    if (!h->measured) return W2D_ERR_STATE;
+    const long nc = 2L * h->model.nbands * h->ntau, nl = h->dm.nlev;
+    std::vector<double> sq((size_t)nc * nl);
+    ChainHdr x;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipMemcpy(&x, hdr_of(h, chain), sizeof(x), hipMemcpyDeviceToHost));
+    const char *base = h->chains + (long)chain * h->L.stride + h->L.gacc + 2 * nc * nl * 8;
+    HIPCHK(hipMemcpy(sq.data(), base, sq.size() * 8, hipMemcpyDeviceToHost));
+    // autocorracc_blocks (Accumulators.F90:342-362): sqdiff / (n_l - 1), n_l = n >> l
+    long long n = x.cnt_meas;
+    for (long l = 0; l < nl; ++l, n /= 2)
+        for (long i = 0; i < nc; ++i) blocks[l * nc + i] = sq[l * nc + i] / (n > 1 ? (double)(n - 1) : 0.0);
+    return W2D_OK;
+}
+
+int w2d_ctqmc_init_counters(w2d_ctqmc *h)
+{
+    if (!h) return W2D_ERR_ARG;
     std::vector<ChainHdr> hd;
     int st = read_headers(h, hd);
     if (st) return st;
-    std::vector<int> use(nc);
-    for (int c = 0; c < nc; ++c) use[c] = hd[c].err == 0;
-    const long nel = (L.stride - L.gtau) / 8;
-    int *duse = nullptr;
-    double *dout = nullptr;
-    HIPCHK(hipMalloc(&duse, (size_t)nc * sizeof(int)));
-    if (hipMalloc(&dout, (size_t)ngroups * nel * 8) != hipSuccess) {
-        (void)hipFree(duse);
-        return W2D_ERR_HIP;
-    }
-    std::vector<double> out((size_t)ngroups * nel);
-    bool ok = hipMemcpy(duse, use.data(), (size_t)nc * sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
-    if (ok) {
-        hipLaunchKernelGGL(w2d::group_sum_kernel, dim3((unsigned)((nel + 255) / 256), (unsigned)ngroups), dim3(256), 0,
-                           h->stream, (const char *)h->chains, L.stride, L.gtau, nel, nc, ngroups, duse, dout);
-        ok = hipGetLastError() == hipSuccess && hipStreamSynchronize(h->stream) == hipSuccess &&
-             hipMemcpy(out.data(), dout, out.size() * 8, hipMemcpyDeviceToHost) == hipSuccess;
-    }
-    (void)hipFree(duse);
-    (void)hipFree(dout);
-    if (!ok) return W2D_ERR_HIP;
-    auto put = [&](double *dst, long off, long n) {
-        if (!dst) return;
-        for (int g = 0; g < ngroups; ++g)
-            std::copy(out.begin() + (long)g * nel + (off - L.gtau) / 8,
-                      out.begin() + (long)g * nel + (off - L.gtau) / 8 + n, dst + (long)g * n);
-    };
-    put(gtau, L.gtau, (long)nf * h->ntau);
-    put(gleg, L.gleg, (long)nf * h->nleg);
-    put(dm_blocks, L.dm, h->dm.dmsize);
-    put(histo, L.histo, (long)nf * (h->maxhisto + 1));
-    put(outer, L.outer, h->model.nsst);
-    for (int g = 0; g < ngroups; ++g) {
-        const int c0 = (int)((long)g * nc / ngroups), c1 = (int)((long)(g + 1) * nc / ngroups);
-        double S[14] = {0};
-        int n = 0;
-        for (int c = c0; c < c1; ++c) {
-            if (!use[c]) continue;
-            const ChainHdr &x = hd[c];
-            const double v[14] = {x.meas_sign, (double)x.cnt_meas, (double)x.acc_add, (double)x.try_add,
-                                  (double)x.acc_rem, (double)x.try_rem, (double)x.acc_glob, (double)x.try_glob,
-                                  (double)x.acc_shift, (double)x.try_shift, (double)x.moves, (double)x.qn_rej,
-                                  x.alg_flops, x.alg_bytes};
-            for (int i = 0; i < 14; ++i) S[i] += v[i];
-            ++n;
+    for (auto &x : hd) reset_counters(x);
+    if ((st = write_headers(h, hd)) || (st = clear_accpair(h))) return st;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return W2D_OK;
+}
+
+int w2d_ctqmc_calibrate(w2d_ctqmc *h, int list, int phase1pairnum, int phase2taugrid, int phase2pairnum,
+                        int nprogress)
+{
+    if (!h || nprogress < 1 || phase2pairnum < 0) return W2D_ERR_ARG;
+    if (list ? (phase1pairnum < 1 || phase1pairnum > h->L.apt_cap)
+             : (phase2taugrid < 1 || phase2taugrid > h->L.naccpair_cap))
+        return W2D_ERR_ARG;
+    h->moved = true;
+    h->aborted = 0;
+    std::vector<ChainHdr> hd;
+    int st = read_headers(h, hd);
+    if (st) return st;
+    for (auto &x : hd) {
+        x.pos = 0;
+        x.halt = w2d::HALT_NONE;
+        if (list) {
+            x.apt_n = phase1pairnum;   // AccPairTau(phase1pairnum), filled from apt_index
+        } else {
+            // AccPair(NBands, 2, phase2taugrid) over [0, taudiff_max] (CTQMC.F90:8103-8108)
+            x.apt_n = 0;
+            x.naccpair = phase2taugrid;
+            x.accpair_max = x.taudiff_max;
         }
-        if (scalars) std::copy(S, S + 14, scalars + 14L * g);
-        if (nused) nused[g] = n;
     }
+    if ((st = write_headers(h, hd))) return st;
+    if (!list && (st = clear_accpair(h))) return st;
+    h->calib = make_int3(list ? 1 : 2, std::min(1000, nprogress), phase2pairnum);
+    const long long per = 4000;
+    long long from = 0;
+    h->last_launches = 0;
+    for (;;) {
+        launch_range(h, from, from + per, 0, per, -1, nullptr, 0);
+        if (hipGetLastError() != hipSuccess) {
+            h->calib = make_int3(0, 1, 0);
+            return W2D_ERR_HIP;
+        }
+        if ((st = read_headers(h, hd))) break;
+        bool again = false;
+        long long f2;
+        if ((st = resume_halted(h, hd, f2, again))) break;
+        bool running = again;
+        for (const auto &x : hd)
+            if (x.err == 0 && x.halt == w2d::HALT_NONE) running = true;
+        if (!running || h->aborted) break;
+        long long lo = -1;
+        for (const auto &x : hd)
+            if (x.err == 0 && x.halt == w2d::HALT_NONE) lo = lo < 0 ? x.pos : std::min(lo, x.pos);
+        from = lo < 0 ? from + per : lo;
+    }
+    h->calib = make_int3(0, 1, 0);
+    if (st) return st;
+    return batch_status(hd);
+}
+
+int w2d_ctqmc_get_pair_stats(w2d_ctqmc *h, int chain, double *accpairtau, int *napt, double *accpair,
+                             int *naccpair)
+{
+    if (!h || chain < 0 || chain >= h->nchains) return W2D_ERR_ARG;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    ChainHdr x;
+    HIPCHK(hipMemcpy(&x, hdr_of(h, chain), sizeof(x), hipMemcpyDeviceToHost));
+    const char *base = h->chains + (long)chain * h->L.stride;
+    const int n = std::min(x.apt_index - 1, x.apt_n);
+    if (napt) *napt = std::max(n, 0);
+    if (naccpair) *naccpair = x.naccpair;
+    if (accpairtau && n > 0) HIPCHK(hipMemcpy(accpairtau, base + h->L.aptau, (size_t)n * 8, hipMemcpyDeviceToHost));
+    if (accpair) {
+        const int nf = 2 * h->model.nbands;
+        HIPCHK(hipMemcpy2D(accpair, (size_t)x.naccpair * 8, base + h->L.accpair, (size_t)h->L.naccpair_cap * 8,
+                           (size_t)x.naccpair * 8, (size_t)nf, hipMemcpyDeviceToHost));
+    }
+    return W2D_OK;
+}
+
+int w2d_ctqmc_get_group_histograms(w2d_ctqmc *h, int ngroups, double *outer_state, double *sign_sst,
+                                   double *sign_state, double *contrib_sst, double *contrib_state, double *accpair)
+{
+    if (!h || ngroups < 1 || ngroups > h->nchains) return W2D_ERR_ARG;
+    if (!h->measured) return W2D_ERR_STATE;
+    void *work = nullptr;
+    HIPCHK(hipMalloc(&work, group_sums_bytes(h, ngroups)));
+    int st = group_sums_into(h, ngroups, work, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+    const w2d::ChainLayout &L = h->L;
+    const long nel = (L.stride - L.gtau) / 8;
+    const double *part = static_cast<const double *>(work);
+    auto put = [&](double *dst, long off, long n) -> int {
+        if (!dst || n <= 0) return W2D_OK;
+        HIPCHK(hipMemcpy2D(dst, (size_t)n * 8, part + (off - L.gtau) / 8, (size_t)nel * 8, (size_t)n * 8,
+                           (size_t)ngroups, hipMemcpyDeviceToHost));
+        return W2D_OK;
+    };
+    const long nst = h->model.nstates, nss = h->model.nsst;
+    if (!st) st = put(outer_state, L.ohst, nst);
+    if (!st) st = put(sign_sst, L.shss, nss);
+    if (!st) st = put(sign_state, L.shst, nst);
+    if (!st) st = put(contrib_sst, L.tcss, nss);
+    if (!st) st = put(contrib_state, L.tcst, nst);
+    if (!st) st = put(accpair, L.accpair, 2L * h->model.nbands * L.naccpair_cap);
+    (void)hipFree(work);
+    return st;
+}
+
+int w2d_ctqmc_get_steps(w2d_ctqmc *h, int chain, int *nmstep, double *gtau_mean_step, double *gtau_mid_step,
+                        double *sign_step)
+{
+    if (!h || chain < 0 || chain >= h->nchains || !nmstep) return W2D_ERR_ARG;
+    *nmstep = h->dm.nmstep;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    const char *base = h->chains + (long)chain * h->L.stride;
+    const size_t nf = 2 * (size_t)h->model.nbands, n = (size_t)h->dm.nmstep;
+    if (gtau_mean_step && h->dm.step_mean) HIPCHK(hipMemcpy(gtau_mean_step, base + h->L.gmstep, n * nf * 8, hipMemcpyDeviceToHost));
+    if (gtau_mid_step && h->dm.step_mid) HIPCHK(hipMemcpy(gtau_mid_step, base + h->L.gdstep, n * nf * 8, hipMemcpyDeviceToHost));
+    if (sign_step && h->dm.step_sign) HIPCHK(hipMemcpy(sign_step, base + h->L.sstep, n * 8, hipMemcpyDeviceToHost));
     return W2D_OK;
 }
 
@@ -1011,6 +1232,26 @@ int w2d_ctqmc_chain_status(const w2d_ctqmc *h, int32_t *status, int32_t *kfmax, 
         if (status) status[c] = hd[c].err ? map_err(hd[c].err) : W2D_OK;
         if (kfmax)
             for (int f = 0; f < nf; ++f) kfmax[(long)c * nf + f] = hd[c].kfmax[f];
+    }
+    return W2D_OK;
+}
+
+int w2d_ctqmc_op_table_bytes(const w2d_ctqmc *h, int64_t *bytes)
+{
+    if (!h || !bytes) return W2D_ERR_ARG;
+    *bytes = (int64_t)h->dm.nops * 8;
+    return W2D_OK;
+}
+
+int w2d_ctqmc_get_orders(const w2d_ctqmc *h, int32_t *nop, int32_t *outer_sst)
+{
+    if (!h || !nop) return W2D_ERR_ARG;
+    std::vector<ChainHdr> hd;
+    int st = read_headers(const_cast<w2d_ctqmc *>(h), hd);
+    if (st) return st;
+    for (int c = 0; c < h->nchains; ++c) {
+        nop[c] = hd[c].nop;
+        if (outer_sst) outer_sst[c] = hd[c].outer;
     }
     return W2D_OK;
 }

@@ -405,8 +405,9 @@ DEV void swap_lists(Ctx &c)
 // ---------------------------------------------------------------------------
 // algorithmic work of one operator application on `no` outer states: a
 // (nt x ns) block times an (ns x no) state matrix plus the diagonal propagator;
-// bytes = operator block + states in and out.  Accumulated in registers during
-// a walk and added to the chain header once.
+// bytes = states in and out (the operator blocks are shared by all chains and
+// charged once per launch by the host).  Accumulated in registers during a
+// walk and added to the chain header once.
 struct Work {
     long long ntns = 0, nss = 0, nts = 0;
     int nout = 0;
@@ -420,7 +421,7 @@ struct Work {
     DEV void commit(Ctx &c) const
     {
         c.wflops += (double)(nout * (2 * ntns + nss));
-        c.wbytes += (double)(8 * ntns + 8LL * nout * (nss + nts));
+        c.wbytes += (double)(8LL * nout * (nss + nts));
     }
 };
 
@@ -1711,6 +1712,20 @@ DEV void commit_window(Ctx &c, int nl)
     swap_lists(c);
 }
 
+// An accepted insertion or removal of a pair of flavour f spanning dt
+// (CTQMC.F90:3579-3587, 4119-4127): the histogram AccPair over NAccPair bins
+// of [0, AccPairMax] and, while the calibration list is open, the distance.
+DEV void record_pair(Ctx &c, int f, double dt)
+{
+    if (lane_id() == 0) {
+        double *ap = reinterpret_cast<double *>(c.base + c.L.accpair) + (long)f * c.L.naccpair_cap;
+        const int bin = min(c.h.naccpair, (int)floor(dt / c.h.accpair_max * (double)c.h.naccpair) + 1);
+        ap[bin - 1] += 1.0;
+        if (c.h.apt_index <= c.h.apt_n) reinterpret_cast<double *>(c.base + c.L.aptau)[c.h.apt_index - 1] = dt;
+    }
+    if (c.h.apt_index <= c.h.apt_n) c.h.apt_index++;
+}
+
 // StepAdd (CTQMC.F90:3353-3612), change_outer = false
 template <int NM>
 DEV int step_add(Ctx &c)
@@ -1829,6 +1844,7 @@ DEV int step_add(Ctx &c)
     wsync();
     c.h.kf[f] = k + 1;
     c.h.kfmax[f] = max(c.h.kfmax[f], k + 1);
+    record_pair(c, f, t2o - t1);
     c.h.det_log += dr_log;
     c.h.det_sign *= dr_sign;
     c.h.trace_log = tl;
@@ -1968,6 +1984,7 @@ DEV int step_rem(Ctx &c)
     block_move(c.fa + fo, fp_a, k, -1);
     block_move(fasf, fp_a, k, -1);
     c.h.kf[f] = k - 1;
+    record_pair(c, f, t2 - t1);
     c.h.det_log += dr_log;
     c.h.det_sign *= dr_sign;
     c.h.trace_log = tl;
@@ -2469,6 +2486,7 @@ struct GtauArgs {
     const double *fc, *fa, *minv;
     const int *fcs, *fas;
     double *gtau, *gleg;
+    double *mstep, *dstep;   // this measurement's row of Gtau_mean_step / Gtau_mid_step (or null)
 };
 
 // One pass over the pairs of flavour f for the Legendre orders [l0, l0 + 128):
@@ -2508,7 +2526,16 @@ DEV void gtau_pass(const GtauArgs &a, int f, int l0, double (&acc)[8])
                 }
                 Mv[u] = m * a.cfgsign;
                 if (idt > a.ntau) idt = a.ntau;
-                if (l0 == 0) global_add(&a.gtau[(long)f * a.ntau + idt - 1], Mv[u]);
+                if (l0 == 0) {
+                    global_add(&a.gtau[(long)f * a.ntau + idt - 1], Mv[u]);
+                    // step-resolved sums (CTQMC.F90:2394-2401): M * cfgsign of
+                    // every pair, and of pairs with |ta - tc| / beta in [0.4, 0.6]
+                    if (a.mstep) global_add(&a.mstep[f], Mv[u] * a.cfgsign);
+                    if (a.dstep) {
+                        const double r = fabs(ta - tc) / a.beta;
+                        if (r >= 0.4 && r <= 0.6) global_add(&a.dstep[f], Mv[u] * a.cfgsign);
+                    }
+                }
             }
             pl[u] = 0.0;
             plm1[u] = 0.0;
@@ -2589,6 +2616,41 @@ DEV double2 gtau_all(const GtauArgs &a)
 __device__ __attribute__((noinline)) double2 meas_gtau(const GtauArgs a) { return gtau_all<true>(a); }
 __device__ __attribute__((noinline)) double2 meas_gtau_narrow(const GtauArgs a) { return gtau_all<false>(a); }
 
+// gtau_accum%add() of the binning accumulator (AccumGtau = 2,
+// autocorracc_add, Accumulators.F90:298-340): this measurement's G(tau) bins
+// (gbuf, filled by atomics) go into the sum and into level 0 by the Welford
+// update; when the low bits of the count n are ones the finished bin of each
+// level (mean of its halves) rolls over to the next.  Lanes over components;
+// the roll-over depth depends on n only, so it is uniform.
+DEV void autocorr_add(const DevModel &M, double *gbuf, double *gacc, double *gtau, long long n0)
+{
+    const long nc = (long)M.nf * M.ntau;
+    double *spill = gacc, *mean = gacc + nc * M.nlev, *sq = gacc + 2 * nc * M.nlev;
+    wsync();
+    for (long i = lane_id(); i < nc; i += WAVE) {
+        // the bins were written by atomics at L2: read past the vector L1
+        double v = __hip_atomic_load(gbuf + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        gbuf[i] = 0.0;
+        gtau[i] += v;
+        long long n = n0;
+        spill[i] += v;
+        double d1 = v - mean[i];
+        mean[i] += d1 / (double)(n + 1);
+        sq[i] += d1 * (v - mean[i]);
+        for (int lvl = 1; (n & 1) != 0 && lvl < M.nlev; ++lvl) {
+            n >>= 1;
+            const long lo = (long)(lvl - 1) * nc + i, hi = (long)lvl * nc + i;
+            const double u = spill[lo] / 2.0;
+            spill[lo] = 0.0;
+            spill[hi] += u;
+            const double e1 = u - mean[hi];
+            mean[hi] += e1 / (double)(n + 1);
+            sq[hi] += e1 * (u - mean[hi]);
+        }
+    }
+    wsync();
+}
+
 // MeasDensityMatrix_beta_half (CTQMC.F90:3088-3151) with update_b2_states_eb
 // (Trace.F90:6213-6469): ket from 0 to beta/2, bra from beta to beta/2
 template <int NM>
@@ -2631,13 +2693,34 @@ DEV void meas_dm(Ctx &c, double *dm)
     }
 }
 
-// MeasSign, MeasOuterHistograms (visits), MeasExpOrder (CTQMC.F90:3264-3350)
+// MeasSign, MeasOuterHistograms, MeasExpOrder (CTQMC.F90:3264-3350): visits
+// of the outer superstate and of each of its states, superstate sign, state
+// signs (trace sign x part-trace sign x configuration sign), trace
+// contributions exp(log part trace) per state and summed per superstate, and
+// the order histogram
 DEV void meas_misc(Ctx &c, double *histo, double *outer_hist)
 {
     const DevModel &M = c.M;
+    const int outer = c.h.outer, no = c.sz[outer], off = c.off[outer];
+    double *ohst = reinterpret_cast<double *>(c.base + c.L.ohst);
+    double *shss = reinterpret_cast<double *>(c.base + c.L.shss);
+    double *shst = reinterpret_cast<double *>(c.base + c.L.shst);
+    double *tcss = reinterpret_cast<double *>(c.base + c.L.tcss);
+    double *tcst = reinterpret_cast<double *>(c.base + c.L.tcst);
     c.h.meas_sign += c.h.cfgsign;
+    double tc = 0.0;
+    for (int i = lane_id(); i < no; i += WAVE) {
+        const double e = exp(c.part[2 * i]);
+        ohst[off + i] += 1.0;
+        shst[off + i] += c.h.trace_sign * c.part[2 * i + 1] * c.h.cfgsign;
+        tcst[off + i] += e;
+        tc += e;
+    }
+    tc = wave_sum(tc);
     if (lane_id() == 0) {
-        outer_hist[c.h.outer] += 1.0;
+        outer_hist[outer] += 1.0;
+        shss[outer] += c.h.cfgsign;
+        tcss[outer] += tc;
         for (int f = 0; f < M.nf; ++f) {
             int k = c.h.kf[f];
             if (k > M.maxhisto) k = M.maxhisto;
@@ -2646,6 +2729,7 @@ DEV void meas_misc(Ctx &c, double *histo, double *outer_hist)
     }
     wsync();
 }
+
 
 // ---------------------------------------------------------------------------
 // kernels
@@ -2789,7 +2873,7 @@ DEV void save_ctx(Ctx &c)
 template <int NM>
 __global__ void __launch_bounds__(WAVE) sweep_kernel(DevModel M, ChainLayout L, char *chains, long long target,
                                                      long long ncorr, int only, w2d_move_record *rec,
-                                                     long long rec_base, const int *abort_word)
+                                                     long long rec_base, const int *abort_word, int3 calib)
 {
     const int chain = blockIdx.x;
     if (only >= 0 && chain != only) return;
@@ -2801,6 +2885,8 @@ __global__ void __launch_bounds__(WAVE) sweep_kernel(DevModel M, ChainLayout L, 
     double *dm = reinterpret_cast<double *>(base + L.dm);
     double *histo = reinterpret_cast<double *>(base + L.histo);
     double *ohist = reinterpret_cast<double *>(base + L.outer);
+    double *gbuf = reinterpret_cast<double *>(base + L.gbuf);
+    double *gacc = reinterpret_cast<double *>(base + L.gacc);
     while (c.h.pos < target && c.h.err == 0 && c.h.halt == HALT_NONE) {
         if (!room_for_move(c)) {
             c.h.halt = HALT_CAP;
@@ -2809,6 +2895,23 @@ __global__ void __launch_bounds__(WAVE) sweep_kernel(DevModel M, ChainLayout L, 
         int kind, out;
         one_move<NM>(c, kind, out);
         const long long m = c.h.pos++;
+        // ctqmc_calibrate (CTQMC.F90:8128-8175): every calib.y moves, phase 1
+        // (calib.x = 1) ends once the distance list is full, phase 2
+        // (calib.x = 2) once every flavour has calib.z accepted pairs
+        if (calib.x && (m + 1) % calib.y == 0) {
+            bool done = calib.x == 1 && c.h.apt_index > c.h.apt_n;
+            if (calib.x == 2) {
+                const double *ap = reinterpret_cast<const double *>(base + L.accpair);
+                int short_f = 0;
+                for (int f = 0; f < M.nf; ++f) {
+                    double n = 0.0;
+                    for (int b = lane_id(); b < c.h.naccpair; b += WAVE) n += ap[(long)f * L.naccpair_cap + b];
+                    short_f += wave_sum(n) < (double)calib.z;
+                }
+                done = short_f == 0;
+            }
+            if (done) c.h.halt = HALT_DONE;
+        }
         if (rec && lane_id() == 0) {
             w2d_move_record &r = rec[m - rec_base];
             r.kind = kind;
@@ -2833,10 +2936,18 @@ __global__ void __launch_bounds__(WAVE) sweep_kernel(DevModel M, ChainLayout L, 
             ga.nf = M.nf, ga.ntau = M.ntau, ga.nleg = M.nleg, ga.kcap = L.kcap;
             ga.beta = M.beta, ga.cfgsign = c.h.cfgsign, ga.legc = c.legc;
             ga.kf = c.h.kf, ga.fc = c.fc, ga.fa = c.fa, ga.minv = c.minv, ga.fcs = c.fcs, ga.fas = c.fas;
-            ga.gtau = gtau, ga.gleg = gleg;
+            ga.gtau = M.accum == 2 ? gbuf : gtau, ga.gleg = gleg;
+            const long long mi = c.h.cnt_meas - 1;   // g_inmeas - 1
+            const bool st = mi < M.nmstep;
+            ga.mstep = (st && M.step_mean) ? reinterpret_cast<double *>(base + L.gmstep) + mi * M.nf : nullptr;
+            ga.dstep = (st && M.step_mid) ? reinterpret_cast<double *>(base + L.gdstep) + mi * M.nf : nullptr;
+            // sign_step is set when the configuration has a pair (CTQMC.F90:2402)
+            if (st && M.step_sign && c.h.nop > 0 && lane_id() == 0)
+                reinterpret_cast<double *>(base + L.sstep)[mi] = c.h.cfgsign;
             const double2 wk = NM == 1 ? meas_gtau_narrow(ga) : meas_gtau(ga);
             c.wflops += wk.x;
             c.wbytes += wk.y;
+            if (M.accum == 2) autocorr_add(M, gbuf, gacc, gtau, c.h.cnt_meas - 1);
             if (__hip_atomic_load(abort_word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)) c.h.halt = HALT_ABORT;
         }
     }
@@ -2878,20 +2989,24 @@ __global__ void __launch_bounds__(WAVE) migrate_kernel(ChainLayout A, ChainLayou
             md[i + j * B.kcap] = ms[e];
         }
     }
+    // boundary caches: as many as both layouts hold (a new window count
+    // invalidates them anyway: the host sets cache_pos = -1)
+    const long nbd = min(A.nbound, B.nbound);
     copy_words(d + B.kst, s + A.kst, (long)nm * nm * 8);
-    copy_words(d + B.bnd, s + A.bnd, (long)A.nbound * nm * nm * 8);
-    copy_words(d + B.bndlog, s + A.bndlog, (long)A.nbound * 8);
-    copy_words(d + B.bndsst, s + A.bndsst, (long)A.nbound * 4);
+    copy_words(d + B.bnd, s + A.bnd, nbd * nm * nm * 8);
+    copy_words(d + B.bndlog, s + A.bndlog, nbd * 8);
+    copy_words(d + B.bndsst, s + A.bndsst, nbd * 4);
     copy_words(d + B.part, s + A.part, 2L * nm * 8 + 16);
     copy_words(d + B.gtau, s + A.gtau, accbytes);
 }
 
 // Accumulators (the contiguous tail [gtau, stride) of each chain block, as
 // doubles) summed over contiguous groups of chains: group g holds chains
-// [g*nc/ng, (g+1)*nc/ng) whose use[c] is set.  Grid: x over elements, y over
-// groups; consecutive threads read consecutive doubles of one chain.
+// [g*nc/ng, (g+1)*nc/ng) that have not stopped on an error (header err == 0).
+// Grid: x over elements, y over groups; consecutive threads read consecutive
+// doubles of one chain.
 __global__ void __launch_bounds__(256) group_sum_kernel(const char *chains, long stride, long acc_off, long nel,
-                                                        int nc, int ng, const int *use, double *out)
+                                                        int nc, int ng, const ChainHdr *hdrs, double *out)
 {
     const long e = (long)blockIdx.x * 256 + threadIdx.x;
     const int g = blockIdx.y;
@@ -2899,8 +3014,37 @@ __global__ void __launch_bounds__(256) group_sum_kernel(const char *chains, long
     const int c0 = (int)((long)g * nc / ng), c1 = (int)((long)(g + 1) * nc / ng);
     double acc = 0.0;
     for (int c = c0; c < c1; ++c)
-        if (use[c]) acc += reinterpret_cast<const double *>(chains + (long)c * stride + acc_off)[e];
+        if (hdrs[c].err == 0) acc += reinterpret_cast<const double *>(chains + (long)c * stride + acc_off)[e];
     out[(long)g * nel + e] = acc;
+}
+
+// The header scalars of the same groups: sum of signs, measurements,
+// accepted / tried moves of each kind, attempted and QN-rejected moves,
+// algorithmic flops and bytes (scal[g][14]), and the chains summed (used[g]).
+// One thread per group.
+__global__ void __launch_bounds__(64) group_scalars_kernel(const ChainHdr *hdrs, int nc, int ng, double *scal,
+                                                           int *used)
+{
+    const int g = blockIdx.x * 64 + threadIdx.x;
+    if (g >= ng) return;
+    const int c0 = (int)((long)g * nc / ng), c1 = (int)((long)(g + 1) * nc / ng);
+    double S[14];
+    for (int i = 0; i < 14; ++i) S[i] = 0.0;
+    int n = 0;
+    for (int c = c0; c < c1; ++c) {
+        const ChainHdr &x = hdrs[c];
+        if (x.err != 0) continue;
+        S[0] += x.meas_sign, S[1] += (double)x.cnt_meas;
+        S[2] += (double)x.acc_add, S[3] += (double)x.try_add;
+        S[4] += (double)x.acc_rem, S[5] += (double)x.try_rem;
+        S[6] += (double)x.acc_glob, S[7] += (double)x.try_glob;
+        S[8] += (double)x.acc_shift, S[9] += (double)x.try_shift;
+        S[10] += (double)x.moves, S[11] += (double)x.qn_rej;
+        S[12] += x.alg_flops, S[13] += x.alg_bytes;
+        ++n;
+    }
+    for (int i = 0; i < 14; ++i) scal[(long)g * 14 + i] = S[i];
+    used[g] = n;
 }
 
 // Initial state of every chain: empty configuration, initial outer superstate

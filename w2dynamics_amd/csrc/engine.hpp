@@ -25,6 +25,9 @@ struct DevModel {
     double p_shift, p_glob, p_outer, p_4op;
     long long nslide;
     int ntau, nleg, maxhisto, has_szt, meas_dm;
+    int accum, nlev;                    // AccumGtau (1 mean, 2 binning) and its levels
+    int nmstep;                         // length of the step-resolved arrays (Nmeas), 0 = off
+    int step_mean, step_mid, step_sign; // Gtau_mean_step, Gtau_mid_step, sign_step
 };
 
 // per-chain scalar state (kept contiguously for all chains, after the chain blocks)
@@ -43,13 +46,15 @@ struct ChainHdr {
     double alg_flops, alg_bytes;   // algorithmic work of this chain's moves and measurements
     long long pos;                 // moves done in the current warmup / measure call
     int halt;                      // HALT_* : why the chain stopped before its target
-    int pad_;
+    int naccpair;                  // bins of the accepted-pair histogram (NAccPair)
+    double accpair_max;            // its range (AccPairMax)
+    int apt_index, apt_n;          // next entry / size of the accepted-pair distance list (AccPairTau)
     int kf[MAXF];
     int kfmax[MAXF];               // largest order each flavour has reached
 };
 
 // reasons a chain stops before the launch's target position (chain state intact)
-enum { HALT_NONE = 0, HALT_CAP = 1, HALT_ABORT = 2 };
+enum { HALT_NONE = 0, HALT_CAP = 1, HALT_ABORT = 2, HALT_DONE = 3 };
 
 // byte offsets of the per-chain arrays inside one chain block
 struct ChainLayout {
@@ -68,6 +73,14 @@ struct ChainLayout {
     long kwork;           // double[2][nmax*nmax]: work matrices (large blocks)
     long ltau, lcode;     // trial operator list (double[lcap], int[lcap])
     long gtau, gleg, dm, histo, outer; // accumulators
+    long gbuf, gacc;      // AccumGtau = 2: one measurement's G(tau); binning levels [3][nlev][nf*ntau]
+    // MeasOuterHistograms (CTQMC.F90:3285-3339): visits of each outer state,
+    // signs of outer superstates / states, trace contributions of both
+    long ohst, shss, shst, tcss, tcst;
+    long accpair;         // accepted pairs per flavour and distance bin [nf][naccpair_cap]
+    long gmstep, gdstep, sstep;   // step-resolved G(tau) mean / mid, sign [nmstep][nf], [nmstep]
+    long aptau;           // accepted-pair distances [apt_cap] (calibration)
+    int naccpair_cap, apt_cap;
     int cap, kcap, nbound, lcap;
     long kmat;            // max(kcap*kcap, nmax*nmax)
     // dynamic LDS carve-up (byte offsets; l_ops / l_k = -1: kept in global memory)

@@ -83,6 +83,14 @@ def load_library(path: str | None = None):
     lib.w2d_ctqmc_set_abort.argtypes = [P, I]
     lib.w2d_ctqmc_aborted.argtypes = [P, C.POINTER(I)]
     lib.w2d_ctqmc_trap_signals.argtypes = []
+    lib.w2d_ctqmc_get_gtau_blocks.argtypes = [P, I, C.POINTER(I), P]
+    lib.w2d_ctqmc_get_orders.argtypes = [P, P, P]
+    lib.w2d_ctqmc_op_table_bytes.argtypes = [P, C.POINTER(C.c_int64)]
+    lib.w2d_ctqmc_init_counters.argtypes = [P]
+    lib.w2d_ctqmc_calibrate.argtypes = [P, I, I, I, I, I]
+    lib.w2d_ctqmc_get_pair_stats.argtypes = [P, I, P, C.POINTER(I), P, C.POINTER(I)]
+    lib.w2d_ctqmc_get_group_histograms.argtypes = [P, I, P, P, P, P, P, P]
+    lib.w2d_ctqmc_get_steps.argtypes = [P, I, C.POINTER(I), P, P, P]
     lib.w2d_ctqmc_version.restype = C.c_char_p
     _lib = lib
     return lib
@@ -169,6 +177,62 @@ class CtqmcGpu:
         cap = C.c_int()
         _check(self.lib.w2d_ctqmc_chain_status(self.h, _p(st), _p(km), C.byref(cap)), "chain_status")
         return dict(status=st, kfmax=km, capacity=cap.value)
+
+    def init_counters(self):
+        """init_counters: move counters, AccPair and the calibration list index."""
+        _check(self.lib.w2d_ctqmc_init_counters(self.h), "init_counters")
+
+    def calibrate(self, list_phase: bool, phase1pairnum: int = 1000, phase2taugrid: int = 100,
+                  phase2pairnum: int = 1000, nprogress: int = 100000):
+        """ctqmc_calibrate for every chain (each stops on its own condition)."""
+        self._check_batch(self.lib.w2d_ctqmc_calibrate(self.h, 1 if list_phase else 0, int(phase1pairnum),
+                                                       int(phase2taugrid), int(phase2pairnum),
+                                                       int(nprogress)), "calibrate")
+
+    def pair_stats(self, chain: int = 0) -> dict:
+        """AccPairTau (distances of calibration phase 1) and AccPair [flavour][bin]."""
+        n, nb = C.c_int(), C.c_int()
+        _check(self.lib.w2d_ctqmc_get_pair_stats(self.h, chain, None, C.byref(n), None, C.byref(nb)),
+               "pair_stats")
+        tau = np.zeros(max(n.value, 1))
+        ap = np.zeros((2 * self.nbands, max(nb.value, 1)))
+        _check(self.lib.w2d_ctqmc_get_pair_stats(self.h, chain, _p(tau), C.byref(n), _p(ap), C.byref(nb)),
+               "pair_stats")
+        return dict(accpairtau=tau[:n.value], accpair=ap[:, :nb.value])
+
+    def group_histograms(self, ngroups: int) -> dict:
+        """Outer-state, sign and trace-contribution histograms and AccPair summed
+        over groups of chains (raw)."""
+        ng, nst, nss = int(ngroups), self.nstates, self.nsst
+        cap = max(self.ntau, 1024)
+        out = dict(outer_state=np.zeros((ng, nst)), sign_sst=np.zeros((ng, nss)),
+                   sign_state=np.zeros((ng, nst)), contrib_sst=np.zeros((ng, nss)),
+                   contrib_state=np.zeros((ng, nst)), accpair=np.zeros((ng, 2 * self.nbands, cap)))
+        _check(self.lib.w2d_ctqmc_get_group_histograms(
+            self.h, ng, *[_p(out[k]) for k in ("outer_state", "sign_sst", "sign_state", "contrib_sst",
+                                              "contrib_state", "accpair")]), "group_histograms")
+        return out
+
+    def steps(self, chain: int = 0) -> dict:
+        """Gtau_mean_step, Gtau_mid_step [Nmeas][flavour] and sign_step [Nmeas] (raw)."""
+        n = C.c_int()
+        _check(self.lib.w2d_ctqmc_get_steps(self.h, chain, C.byref(n), None, None, None), "steps")
+        nf = 2 * self.nbands
+        m, d, sg = np.zeros((max(n.value, 1), nf)), np.zeros((max(n.value, 1), nf)), np.zeros(max(n.value, 1))
+        _check(self.lib.w2d_ctqmc_get_steps(self.h, chain, C.byref(n), _p(m), _p(d), _p(sg)), "steps")
+        return dict(gtau_mean_step=m[:n.value], gtau_mid_step=d[:n.value], sign_step=sg[:n.value])
+
+    def model_ops_size(self) -> int:
+        """Doubles in the eigenbasis operator blocks (shared by all chains)."""
+        b = C.c_int64()
+        _check(self.lib.w2d_ctqmc_op_table_bytes(self.h, C.byref(b)), "op_table_bytes")
+        return b.value // 8
+
+    def orders(self) -> np.ndarray:
+        """Operators in each chain's configuration (2 x expansion order)."""
+        n = np.zeros(self.nchains, np.int32)
+        _check(self.lib.w2d_ctqmc_get_orders(self.h, _p(n), None), "get_orders")
+        return n
 
     def set_abort(self, flag: bool = True):
         """Raise (or clear) the abort word: sweeps stop at the next measurement
@@ -268,28 +332,41 @@ class CtqmcGpu:
             acc = self.accumulators()
         return normalise(acc, self.nbands, self.beta, self.ntau, self.nleg, self.model())
 
-    def group_sums(self, ngroups: int) -> dict:
+    def gtau_blocks(self, chain: int = 0) -> np.ndarray:
+        """AccumGtau = 2: variances of the means of bins of 2^l measurements of
+        chain ``chain``'s raw G(tau), [level][flavour][ntau] (gtau_blocks)."""
+        nl = C.c_int()
+        _check(self.lib.w2d_ctqmc_get_gtau_blocks(self.h, int(chain), C.byref(nl), None), "gtau_blocks")
+        out = np.zeros(nl.value * 2 * self.nbands * self.ntau)
+        _check(self.lib.w2d_ctqmc_get_gtau_blocks(self.h, int(chain), C.byref(nl), _p(out)), "gtau_blocks")
+        return out.reshape(nl.value, 2 * self.nbands, self.ntau)
+
+    def group_sums(self, ngroups: int, with_dm: bool = True) -> dict:
         """Raw accumulators summed on the GPU over ``ngroups`` contiguous groups
         of chains, leaving out chains stopped on an error; the density matrix
-        comes as its full (nstates x nstates) form per group."""
+        comes as its full (nstates x nstates) form per group (left out, with
+        the occupations that derive from it, when ``with_dm`` is false)."""
         nf, ng = 2 * self.nbands, int(ngroups)
         m = self.model()
         dmsize = int((m["sizes"].astype(np.int64) ** 2).sum())
         g = np.zeros((ng, nf, self.ntau))
         gl = np.zeros((ng, nf, self.nleg))
-        db = np.zeros((ng, dmsize))
+        db = np.zeros((ng, dmsize)) if with_dm else None
         hi = np.zeros((ng, nf, self.maxhisto + 1))
         oh = np.zeros((ng, self.nsst))
         sc = np.zeros((ng, 14))
         nu = np.zeros(ng, np.int32)
-        _check(self.lib.w2d_ctqmc_get_group_sums(self.h, ng, _p(g), _p(gl), _p(db), _p(hi), _p(oh),
-                                                 _p(sc), _p(nu)), "get_group_sums")
-        dm = np.zeros((ng, self.nstates, self.nstates))
-        o = 0
-        for off, n in zip(m["offsets"], m["sizes"]):
-            dm[:, off:off + n, off:off + n] = db[:, o:o + n * n].reshape(ng, n, n).transpose(0, 2, 1)
-            o += n * n
-        return dict(gtau=g, gleg=gl, dm=dm, histo=hi, outer_hist=oh, scalars=sc, nused=nu)
+        _check(self.lib.w2d_ctqmc_get_group_sums(self.h, ng, _p(g), _p(gl), _p(db) if with_dm else None,
+                                                 _p(hi), _p(oh), _p(sc), _p(nu)), "get_group_sums")
+        out = dict(gtau=g, gleg=gl, histo=hi, outer_hist=oh, scalars=sc, nused=nu)
+        if with_dm:
+            dm = np.zeros((ng, self.nstates, self.nstates))
+            o = 0
+            for off, n in zip(m["offsets"], m["sizes"]):
+                dm[:, off:off + n, off:off + n] = db[:, o:o + n * n].reshape(ng, n, n).transpose(0, 2, 1)
+                o += n * n
+            out["dm"] = dm
+        return out
 
     def aggregate(self, ngroups: int = 64) -> dict:
         """Estimates over all chains that completed: the sign-ratio estimate

@@ -12,11 +12,14 @@
 !! Differences from the reference module, all on purpose:
 !!  - one handle runs n_chains independent Markov chains on one GPU (chain c
 !!    seeded with Nseed + c, like MPI ranks NSeed + rank); the result arrays are
-!!    the mean over chains of each chain's normalised observables, i.e. what the
-!!    driver obtains from the mean over ranks.  n_chains = 1 (default) is the
+!!    the normalisation of CTQMC.F90:8663-8771 applied to the sums over all
+!!    chains (the sign-ratio estimate sum(G sign) / sum(sign), finite where a
+!!    single chain's sign sum vanishes).  n_chains = 1 (default) is the
 !!    reference's single-rank run move for move;
 !!  - errors are returned by the library as status codes; this layer prints
-!!    the code and sets `aborted` = 2 instead of stopping the process;
+!!    the code and sets `aborted` = 2 instead of stopping the process; chains
+!!    that stop on an error while others complete are left out of the results
+!!    (a warning, not an abort);
 !!  - only the Z-space sector (iSector = 1) with diagonal hybridisation is
 !!    available; other sectors set aborted = 2.
 !===============================================================================
@@ -122,6 +125,22 @@ module MCTQMC
          integer(c_int), dimension(*), intent(in) :: orb, spin, ca
          real(c_double), dimension(*), intent(out) :: weights
       end function
+      integer(c_int) function w2d_ctqmc_get_group_sums(h, ngroups, gtau, gleg, dmb, histo, outer, &
+                                                       scalars, nused) bind(C, name="w2d_ctqmc_get_group_sums")
+         import :: c_int, c_ptr, c_double, c_int32_t
+         type(c_ptr), value :: h
+         integer(c_int), value :: ngroups
+         real(c_double) :: gtau(*), gleg(*), dmb(*), histo(*), outer(*), scalars(*)
+         integer(c_int32_t) :: nused(*)
+      end function
+      integer(c_int) function w2d_ctqmc_aborted(h, ab) bind(C, name="w2d_ctqmc_aborted")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: h
+         integer(c_int) :: ab
+      end function
+      integer(c_int) function w2d_ctqmc_trap_signals() bind(C, name="w2d_ctqmc_trap_signals")
+         import :: c_int
+      end function
       integer(c_int) function w2d_ctqmc_destroy(h) bind(C, name="w2d_ctqmc_destroy")
          import :: c_int, c_ptr
          type(c_ptr), value :: h
@@ -130,11 +149,15 @@ module MCTQMC
 
 contains
 
-   !> status check: print and flag instead of stopping (the reference stops)
+   !> status check: print and flag instead of stopping (the reference stops);
+   !! W2D_ERR_PARTIAL (some chains stopped on an error, the others completed)
+   !! is a warning: the stopped chains are left out of the results
    logical function failed(st, where)
       integer(c_int), intent(in) :: st
       character(*), intent(in) :: where
-      failed = st /= 0
+      failed = st /= 0 .and. st /= 7
+      if (st == 7) write (0, '(A,A,A)') "MCTQMC: ", where, &
+         ": some chains stopped on an error and are left out of the results"
       if (failed) then
          write (0, '(A,A,A,I0)') "MCTQMC: ", where, " returned status ", st
          aborted = 2
@@ -219,6 +242,8 @@ contains
       bs = param_value("beta")
       read (bs, *) beta
       call load_model()
+      ! register_trap for SIGTERM, SIGINT, SIGUSR1 (CTQMC.F90:8076-8078)
+      if (failed(w2d_ctqmc_trap_signals(), "w2d_ctqmc_trap_signals")) return
       if (.false.) print *, iter_no, screening_function(1, 1, 1, 1, 1)
    end subroutine init_solver
 
@@ -279,6 +304,7 @@ contains
       integer, intent(in) :: iSector, iComponent
       integer(c_int64_t) :: nmeas
       integer(8) :: t0, t1, rate
+      integer(c_int) :: ab
       character(len=:), allocatable :: v
       if (iSector /= 1) then
          write (0, '(A,I0,A,I0)') "MCTQMC: only the Z-space sector is available, got sector ", &
@@ -293,58 +319,64 @@ contains
       call system_clock(t1)
       time_sim = real(t1 - t0, KINDR)/real(rate, KINDR)
       time_sim_steps = time_sim
+      ! a trapped signal ends the sampling early (CTQMC.F90:8648-8658)
+      if (failed(w2d_ctqmc_aborted(handle, ab), "w2d_ctqmc_aborted")) return
+      aborted = ab
+      if (ab == 2) return
       call collect_results()
    end subroutine ctqmc_measure
 
    subroutine collect_results()
-      real(KINDR), allocatable :: g(:,:,:), gl(:,:,:), dm(:,:,:), hi(:,:,:), oh(:,:), sc(:,:)
-      real(KINDR), allocatable :: dmb(:,:), v(:,:), diag(:)
-      real(KINDR) :: cnt, msign, zs, lfac
-      integer :: c, b, s, f, l, k, o, n, b2, s2, voff
+      ! sums over all chains that completed (one group), then the normalisation
+      ! of CTQMC.F90:8663-8771 once: a ratio of sums, finite where one chain's
+      ! sign sum vanishes
+      real(KINDR), allocatable :: g(:,:), gl(:,:), dmb(:), hi(:,:), oh(:), v(:,:), blk(:,:), diag(:)
+      real(KINDR) :: sc(14), cnt, msign, zs, lfac
+      integer(c_int32_t) :: nu(1)
+      integer :: b, s, f, l, k, o, n, b2, s2, voff, boff
       integer(c_int) :: st
-      allocate (g(ntau, 2*nb, n_chains), gl(nleg, 2*nb, n_chains), dm(nstates, nstates, n_chains))
-      allocate (hi(maxhisto + 1, 2*nb, n_chains), oh(nsst, n_chains), sc(14, n_chains))
-      st = w2d_ctqmc_get_accumulators(handle, g, gl, dm, hi, oh, sc)
-      if (failed(st, "w2d_ctqmc_get_accumulators")) return
+      allocate (g(ntau, 2*nb), gl(nleg, 2*nb), dmb(max(1, sum(model_sizes**2))), hi(maxhisto + 1, 2*nb))
+      allocate (oh(nsst))
+      st = w2d_ctqmc_get_group_sums(handle, 1_c_int, g, gl, dmb, hi, oh, sc, nu)
+      if (failed(st, "w2d_ctqmc_get_group_sums")) return
       call reset_results()
       allocate (diag(nstates))
-      do c = 1, n_chains
-         cnt = sc(2, c)
-         msign = sc(1, c)/cnt
-         zs = -1.0_KINDR/(cnt*msign)
-         mean_sign = mean_sign + msign/n_chains
-         do s = 1, 2
-            do b = 1, nb
-               f = b + nb*(s - 1)
-               Gtau(b, s, :) = Gtau(b, s, :) - (g(:, f, c)/cnt)/(-msign)*ntau/beta**2/n_chains
-               do l = 1, nleg
-                  lfac = sqrt(2.0_KINDR*(l - 1) + 1.0_KINDR)/beta
-                  GLeg(b, s, l) = GLeg(b, s, l) - gl(l, f, c)*lfac*zs/n_chains
-               end do
-               Histo(b, s, :) = Histo(b, s, :) + hi(:, f, c)/cnt/n_chains
+      cnt = sc(2)
+      msign = sc(1)/cnt
+      zs = -1.0_KINDR/(cnt*msign)
+      mean_sign = msign
+      do s = 1, 2
+         do b = 1, nb
+            f = b + nb*(s - 1)
+            Gtau(b, s, :) = -(g(:, f)/cnt)/(-msign)*ntau/beta**2
+            do l = 1, nleg
+               lfac = sqrt(2.0_KINDR*(l - 1) + 1.0_KINDR)/beta
+               GLeg(b, s, l) = -gl(l, f)*lfac*zs
             end do
+            Histo(b, s, :) = hi(:, f)/cnt
          end do
-         OuterSuperstateHisto = OuterSuperstateHisto + oh(:, c)/cnt/n_chains
-         ! density matrix: eigenbasis blocks -> occupation basis (V rho V^T)
-         voff = 0
-         do k = 1, nsst
-            o = model_offs(k)
-            n = model_sizes(k)
-            v = reshape(model_vec(voff + 1:voff + n*n), [n, n])
-            dmb = dm(o + 1:o + n, o + 1:o + n, c)*zs
-            DensityMatrix(o + 1:o + n, o + 1:o + n) = DensityMatrix(o + 1:o + n, o + 1:o + n) + &
-                                                      matmul(v, matmul(dmb, transpose(v)))/n_chains
-            voff = voff + n*n
-         end do
-         AccAdd(1) = AccAdd(1) + sc(3, c)/max(sc(4, c), 1.0_KINDR)/n_chains
-         AccRem(1) = AccRem(1) + sc(5, c)/max(sc(6, c), 1.0_KINDR)/n_chains
-         AccGlob = AccGlob + sc(7, c)/max(sc(8, c), 1.0_KINDR)/n_chains
-         AccShift = AccShift + sc(9, c)/max(sc(10, c), 1.0_KINDR)/n_chains
-         TryAdd(1) = TryAdd(1) + int(sc(4, c), c_int64_t)
-         TryRem(1) = TryRem(1) + int(sc(6, c), c_int64_t)
-         TryGlob = TryGlob + int(sc(8, c), c_int64_t)
-         TryShift = TryShift + int(sc(10, c), c_int64_t)
       end do
+      OuterSuperstateHisto = oh/cnt
+      ! density matrix: eigenbasis blocks -> occupation basis (V rho V^T)
+      voff = 0
+      boff = 0
+      do k = 1, nsst
+         o = model_offs(k)
+         n = model_sizes(k)
+         v = reshape(model_vec(voff + 1:voff + n*n), [n, n])
+         blk = reshape(dmb(boff + 1:boff + n*n), [n, n])*zs
+         DensityMatrix(o + 1:o + n, o + 1:o + n) = matmul(v, matmul(blk, transpose(v)))
+         voff = voff + n*n
+         boff = boff + n*n
+      end do
+      AccAdd(1) = sc(3)/max(sc(4), 1.0_KINDR)
+      AccRem(1) = sc(5)/max(sc(6), 1.0_KINDR)
+      AccGlob = sc(7)/max(sc(8), 1.0_KINDR)
+      AccShift = sc(9)/max(sc(10), 1.0_KINDR)
+      TryAdd(1) = int(sc(4), c_int64_t)
+      TryRem(1) = int(sc(6), c_int64_t)
+      TryGlob = int(sc(8), c_int64_t)
+      TryShift = int(sc(10), c_int64_t)
       ! occ(b1,s1,b2,s2) = sum_k rho_kk n_{b1 s1}(k) n_{b2 s2}(k), occupation basis
       do k = 1, nstates
          diag(k) = DensityMatrix(k, k)
