@@ -135,7 +135,7 @@ def thermalise(gpu, ncorr, min_sweeps, chunk, budget_s, tol=0.015):
 
 def run_gpu(cfg, nchains, capacity, min_sweeps, steps, ncorr, seed, therm_s):
     import torch
-    from w2dynamics_amd.ctqmc import CtqmcGpu
+    from w2dynamics_amd.ctqmc import CtqmcGpu, check_accumulators
     prob, tdm = make_problem(cfg, ncorr)
     gpu = CtqmcGpu.from_problem(prob, nchains=nchains, seed=seed, capacity=capacity, device=0)
     gpu.set_taudiffmax(tdm)
@@ -151,6 +151,7 @@ def run_gpu(cfg, nchains, capacity, min_sweeps, steps, ncorr, seed, therm_s):
     order_after = float(gpu.orders().mean())
     ng = min(64, nchains)
     sums = gpu.group_sums(ng, with_dm=False)
+    check_accumulators(sums)       # raises if any G(tau) accumulation was lost
     sc = sums["scalars"]
     q = prob.qmc()
     out = dict(wall=wall, kernel_ms=kms, launches=nlaunch,

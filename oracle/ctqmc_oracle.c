@@ -157,6 +157,13 @@ typedef struct {
     double *sign_hist_sst, *sign_hist_states, *tc_sst, *tc_states;
     double meas_sign;
     long cnt_meas;
+    /* accepted-pair statistics (CTQMC.F90:66-70): AccPair [nf][ACC_CAP],
+     * NAccPair bins of [0, AccPairMax]; AccPairTau list of apt_n entries */
+    double *accpair, accpair_max, *aptau;
+    int naccpair, apt_index, apt_n;
+    /* step-resolved G(tau) mean / mid [nmstep][nf] and sign [nmstep] */
+    double *mstep, *dstep, *sstep;
+    int nmstep, step_mean, step_mid, step_sign;
     /* AccumGtau = 2: binning accumulator of G(tau) (autocorracc) */
     int accum, nlev;
     long acc_n;
@@ -657,6 +664,21 @@ static void accept_trace(ochain_t *ch, olog_t tr)
     memcpy(ch->parttrace, ch->tpart, sizeof(olog_t) * (size_t)no);
 }
 
+#define ACC_CAP 8192
+#define APT_CAP 65536
+
+/* accepted pair of flavour f spanning dt (CTQMC.F90:3579-3587, 4119-4127) */
+static void record_pair(ochain_t *ch, int f, double dt)
+{
+    int bin = (int)floor(dt / ch->accpair_max * (double)ch->naccpair) + 1;
+    if (bin > ch->naccpair) bin = ch->naccpair;
+    ch->accpair[(long)f * ACC_CAP + bin - 1] += 1.0;
+    if (ch->apt_index <= ch->apt_n) {
+        ch->aptau[ch->apt_index - 1] = dt;
+        ch->apt_index++;
+    }
+}
+
 /* StepAdd (CTQMC.F90:3353-3612) with pair_OperAdd (Trace.F90:3657-3698) and
  * process_OperAdd (Trace.F90:4275-4462); change_outer is always false here */
 static int step_add(ochain_t *ch)
@@ -735,6 +757,7 @@ static int step_add(ochain_t *ch)
         accept_trace(ch, trn);
         ch->cfgsign = -config_sign(ch);
         ch->acc_add++;
+        record_pair(ch, f, o2.tau - o1.tau);
         free(Q), free(R);
         return 1;
     }
@@ -806,6 +829,7 @@ static int step_rem(ochain_t *ch)
         accept_trace(ch, trn);
         ch->cfgsign = -config_sign(ch);
         ch->acc_rem++;
+        record_pair(ch, f, o2.tau - o1.tau);
         return 1;
     }
     insert_op(ch, i1, o1);
@@ -1168,6 +1192,14 @@ static void meas_gtau(ochain_t *ch)
             }
             if (idt > ch->ntau) idt = ch->ntau;
             (ch->accum == 2 ? ch->gmeas : ch->gtau_buf)[(long)f * ch->ntau + idt - 1] += M;
+            /* step-resolved sums (CTQMC.F90:2394-2402), g_inmeas = cnt_meas */
+            long mi = ch->cnt_meas - 1;
+            if (mi < ch->nmstep) {
+                if (ch->step_mean) ch->mstep[mi * nf + f] += M * ch->cfgsign;
+                double r = fabs(a->tau - c->tau) / ch->beta;
+                if (ch->step_mid && r >= 0.4 && r <= 0.6) ch->dstep[mi * nf + f] += M * ch->cfgsign;
+                if (ch->step_sign) ch->sstep[mi] = ch->cfgsign;
+            }
         }
     }
     /* gtau_accum%add() (CTQMC.F90:2439) */
@@ -1298,6 +1330,9 @@ void *oracle_create(int nb, int nsst, int nmax, const int *sizes, const int *off
     ch->accum = (int)iparams[6];
     ch->nlev = ilog2_l(iparams[7]);
     if (ch->nlev < 1) ch->nlev = 1;
+    /* iparams[8..10] Gtau_mean_step, Gtau_mid_step, sign_step (arrays of Nmeas) */
+    ch->step_mean = iparams[8] != 0, ch->step_mid = iparams[9] != 0, ch->step_sign = iparams[10] != 0;
+    ch->nmstep = (ch->step_mean || ch->step_mid || ch->step_sign) ? (int)iparams[7] : 0;
     ch->cap = 1024;
     ch->op = malloc(sizeof(oop_t) * (size_t)ch->cap);
     ch->ket = malloc(sizeof(double) * (size_t)nmax);
@@ -1313,6 +1348,14 @@ void *oracle_create(int nb, int nsst, int nmax, const int *sizes, const int *off
     ch->gtau_buf = calloc((size_t)nf * ch->ntau, sizeof(double));
     ch->gleg = calloc((size_t)nf * ch->nleg, sizeof(double));
     ch->gmeas = calloc((size_t)nf * ch->ntau, sizeof(double));
+    ch->accpair = calloc((size_t)nf * ACC_CAP, sizeof(double));
+    ch->aptau = calloc(APT_CAP, sizeof(double));
+    ch->naccpair = ch->ntau < ACC_CAP ? ch->ntau : ACC_CAP;   /* NGtau bins of [0, beta] */
+    ch->accpair_max = beta;
+    ch->apt_index = 1, ch->apt_n = 0;
+    ch->mstep = calloc((size_t)nf * (ch->nmstep + 1), sizeof(double));
+    ch->dstep = calloc((size_t)nf * (ch->nmstep + 1), sizeof(double));
+    ch->sstep = calloc((size_t)ch->nmstep + 1, sizeof(double));
     ch->acc_spill = calloc((size_t)nf * ch->ntau * ch->nlev, sizeof(double));
     ch->acc_mean = calloc((size_t)nf * ch->ntau * ch->nlev, sizeof(double));
     ch->acc_sq = calloc((size_t)nf * ch->ntau * ch->nlev, sizeof(double));
@@ -1360,6 +1403,7 @@ void oracle_destroy(void *h)
     free(ch->klog_b2), free(ch->blog_b2), free(ch->parttrace), free(ch->tpart);
     free(ch->gtau_buf), free(ch->gleg), free(ch->dm), free(ch->histo), free(ch->outer_hist);
     free(ch->gmeas), free(ch->acc_spill), free(ch->acc_mean), free(ch->acc_sq);
+    free(ch->accpair), free(ch->aptau), free(ch->mstep), free(ch->dstep), free(ch->sstep);
     free(ch->sign_hist_sst), free(ch->tc_sst), free(ch->outer_state_hist);
     free(ch->sign_hist_states), free(ch->tc_states);
     free(ch);
@@ -1386,13 +1430,98 @@ void oracle_moves(void *h, long n, omove_rec_t *rec)
     }
 }
 
-/* reset counters (init_counters, CTQMC.F90:847-899) */
+/* reset counters (init_counters, CTQMC.F90:847-899): move counters, AccPair,
+ * the calibration list index */
 void oracle_reset_counters(void *h)
 {
     ochain_t *ch = h;
     ch->try_add = ch->try_rem = ch->try_glob = ch->try_shift = 0;
     ch->acc_add = ch->acc_rem = ch->acc_glob = ch->acc_shift = 0;
     ch->accqn_add = ch->accqn_rem = ch->accqn_glob = 0;
+    memset(ch->accpair, 0, sizeof(double) * (size_t)NF(ch) * ACC_CAP);
+    ch->apt_index = 1;
+}
+
+/* ctqmc_warmup's AccPair over NGtau bins of [0, beta] (CTQMC.F90:8199-8204) */
+void oracle_begin_warmup(void *h)
+{
+    ochain_t *ch = h;
+    ch->naccpair = ch->ntau < ACC_CAP ? ch->ntau : ACC_CAP;
+    ch->accpair_max = ch->beta;
+    oracle_reset_counters(h);
+}
+
+/* ctqmc_calibrate (CTQMC.F90:8088-8178): moves until, checked every
+ * min(1000, nprogress) moves, the distance list holds phase1pairnum entries
+ * (list) or every flavour has phase2pairnum accepted pairs in AccPair over
+ * phase2taugrid bins of [0, taudiff_max] (not list); returns the moves done */
+long oracle_calibrate(void *h, int list, int phase1pairnum, int phase2taugrid, int phase2pairnum, int nprogress)
+{
+    ochain_t *ch = h;
+    if (list) {
+        ch->apt_n = phase1pairnum;
+    } else {
+        ch->apt_n = 0;
+        memset(ch->accpair, 0, sizeof(double) * (size_t)NF(ch) * ACC_CAP);
+        ch->naccpair = phase2taugrid;
+        ch->accpair_max = ch->taudiff_max;
+    }
+    int every = nprogress < 1000 ? nprogress : 1000;
+    long i = 0;
+    for (;;) {
+        ++i;
+        oracle_moves(h, 1, NULL);
+        if (i % every != 0) continue;
+        if (list) {
+            if (ch->apt_index > ch->apt_n) return i;
+        } else {
+            int ok = 1;
+            for (int f = 0; f < NF(ch) && ok; ++f) {
+                double n = 0.0;
+                for (int b = 0; b < ch->naccpair; ++b) n += ch->accpair[(long)f * ACC_CAP + b];
+                if (n < (double)phase2pairnum) ok = 0;
+            }
+            if (ok) return i;
+        }
+    }
+}
+
+/* accepted-pair statistics: the distance list (returns its length) and AccPair
+ * [nf][naccpair] */
+int oracle_pair_stats(void *h, double *aptau, double *accpair, int *naccpair)
+{
+    ochain_t *ch = h;
+    int n = ch->apt_index - 1 < ch->apt_n ? ch->apt_index - 1 : ch->apt_n;
+    if (aptau) memcpy(aptau, ch->aptau, sizeof(double) * (size_t)(n > 0 ? n : 0));
+    if (naccpair) *naccpair = ch->naccpair;
+    if (accpair)
+        for (int f = 0; f < NF(ch); ++f)
+            memcpy(accpair + (long)f * ch->naccpair, ch->accpair + (long)f * ACC_CAP,
+                   sizeof(double) * (size_t)ch->naccpair);
+    return n > 0 ? n : 0;
+}
+
+/* MeasOuterHistograms accumulators (raw) */
+void oracle_outer_histos(void *h, double *outer_state, double *sign_sst, double *sign_state, double *contrib_sst,
+                         double *contrib_state)
+{
+    ochain_t *ch = h;
+    memcpy(outer_state, ch->outer_state_hist, sizeof(double) * (size_t)ch->nstates);
+    memcpy(sign_sst, ch->sign_hist_sst, sizeof(double) * (size_t)ch->nsst);
+    memcpy(sign_state, ch->sign_hist_states, sizeof(double) * (size_t)ch->nstates);
+    memcpy(contrib_sst, ch->tc_sst, sizeof(double) * (size_t)ch->nsst);
+    memcpy(contrib_state, ch->tc_states, sizeof(double) * (size_t)ch->nstates);
+}
+
+/* step-resolved arrays (raw): [nmstep][nf], [nmstep][nf], [nmstep]; returns nmstep */
+int oracle_steps(void *h, double *mstep, double *dstep, double *sstep)
+{
+    ochain_t *ch = h;
+    size_t n = (size_t)ch->nmstep, nf = (size_t)NF(ch);
+    if (mstep) memcpy(mstep, ch->mstep, sizeof(double) * n * nf);
+    if (dstep) memcpy(dstep, ch->dstep, sizeof(double) * n * nf);
+    if (sstep) memcpy(sstep, ch->sstep, sizeof(double) * n);
+    return ch->nmstep;
 }
 
 /* ctqmc_measure's sampling loop: nmeas x (ncorr moves + measurements) */

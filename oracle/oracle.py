@@ -57,6 +57,14 @@ def lib():
         _lib.oracle_gtau_blocks.argtypes = [P, P]
         _lib.oracle_gtau_blocks.restype = C.c_int
         _lib.oracle_autocorr.argtypes = [C.c_long, C.c_int, C.c_long, P, P, P]
+        _lib.oracle_begin_warmup.argtypes = [P]
+        _lib.oracle_calibrate.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
+        _lib.oracle_calibrate.restype = C.c_long
+        _lib.oracle_pair_stats.argtypes = [P, P, P, P]
+        _lib.oracle_pair_stats.restype = C.c_int
+        _lib.oracle_outer_histos.argtypes = [P, P, P, P, P, P]
+        _lib.oracle_steps.argtypes = [P, P, P, P]
+        _lib.oracle_steps.restype = C.c_int
     return _lib
 
 
@@ -115,7 +123,10 @@ class OracleChain:
                                           int("Szt" in quantum_numbers),
                                           int(qmc["MeasDensityMatrix"]),
                                           int(qmc.get("AccumGtau", 1)),
-                                          int(qmc["Nmeas"])], dtype=np.int64)
+                                          int(qmc["Nmeas"]),
+                                          int(qmc.get("Gtau_mean_step", 0)),
+                                          int(qmc.get("Gtau_mid_step", 0)),
+                                          int(qmc.get("sign_step", 0))], dtype=np.int64)
         for key in ("opoff", "evoff"):
             self._keep[key] = np.ascontiguousarray(self._keep[key], dtype=np.int64)
         k = self._keep
@@ -148,8 +159,40 @@ class OracleChain:
         return rec
 
     def warmup(self, n: int):
-        lib().oracle_reset_counters(self.h)
+        lib().oracle_begin_warmup(self.h)
         lib().oracle_moves(self.h, int(n), None)
+
+    def init_counters(self):
+        lib().oracle_reset_counters(self.h)
+
+    def calibrate(self, list_phase: bool, phase1pairnum=1000, phase2taugrid=100, phase2pairnum=1000,
+                  nprogress=100000) -> int:
+        """ctqmc_calibrate; returns the moves it took."""
+        return lib().oracle_calibrate(self.h, 1 if list_phase else 0, int(phase1pairnum), int(phase2taugrid),
+                                      int(phase2pairnum), int(nprogress))
+
+    def pair_stats(self) -> dict:
+        nb = C.c_int()
+        tau = np.zeros(1 << 16)
+        ap = np.zeros(2 * self.nb * 8192)
+        n = lib().oracle_pair_stats(self.h, _p(tau), _p(ap), C.byref(nb))
+        return dict(accpairtau=tau[:n].copy(),
+                    accpair=ap[:2 * self.nb * nb.value].reshape(2 * self.nb, nb.value).copy())
+
+    def outer_histos(self) -> dict:
+        nst, nss = 4 ** self.nb, self.model.nsst
+        out = dict(outer_state=np.zeros(nst), sign_sst=np.zeros(nss), sign_state=np.zeros(nst),
+                   contrib_sst=np.zeros(nss), contrib_state=np.zeros(nst))
+        lib().oracle_outer_histos(self.h, *[_p(out[k]) for k in ("outer_state", "sign_sst", "sign_state",
+                                                                  "contrib_sst", "contrib_state")])
+        return out
+
+    def steps(self) -> dict:
+        n = lib().oracle_steps(self.h, None, None, None)
+        nf = 2 * self.nb
+        m, d, sg = np.zeros((max(n, 1), nf)), np.zeros((max(n, 1), nf)), np.zeros(max(n, 1))
+        lib().oracle_steps(self.h, _p(m), _p(d), _p(sg))
+        return dict(gtau_mean_step=m[:n], gtau_mid_step=d[:n], sign_step=sg[:n])
 
     def measure(self, nmeas: int, ncorr: int):
         lib().oracle_reset_counters(self.h)

@@ -84,6 +84,8 @@ program ref_driver
       call write_results(trim(dir))
    case ('eval')
       call run_eval(trim(dir))
+   case ('calib')
+      call run_calib(trim(dir), nwarm)
    case ('runcfg')
       call start_from_config(trim(dir))
       call ctqmc_warmup(nwarm)
@@ -237,6 +239,61 @@ contains
       close(60)
       close(61)
    end subroutine run_eval
+
+   ! the calibration of cthyb_solver.py:899-944 for one rank (phase 1 list,
+   ! TaudiffMax from its 99.5 % point, phase 2 histogram; the fit of phase 2 is
+   ! the driver's and is not needed here), then warm-up and measurement; writes
+   ! calib.bin: phase-1 list, taudiffmax, phase-2 AccPair, then after the
+   ! measurement AccPair, the outer-state histograms and the step arrays
+   subroutine run_calib(d, nw)
+      character(*), intent(in) :: d
+      integer(c_int64_t), intent(in) :: nw
+      real(c_double), allocatable :: lst(:), lst0(:)
+      real(c_double) :: tdm
+      integer :: n1
+      n1 = 1000
+      call init_counters()
+      call ctqmc_calibrate(.true., n1, 100, 1000, 100000)
+      lst0 = AccPairTau
+      lst = lst0
+      call sort_real(lst)
+      tdm = lst(int(0.995d0*n1) + 1)
+      call set_taudiffmax(tdm)
+      call init_counters()
+      call ctqmc_calibrate(.false., n1, 100, 1000, 100000)
+      open(90, file=d//'/calib.bin', access='stream', form='unformatted', status='replace', action='write')
+      write(90) int(size(lst0), c_int32_t)
+      write(90) lst0
+      write(90) tdm
+      write(90) int(size(AccPair, 3), c_int32_t)
+      write(90) real(AccPair, c_double)
+      call ctqmc_warmup(nw)
+      call ctqmc_measure(1, 1)
+      write(90) int(size(AccPair, 3), c_int32_t)
+      write(90) real(AccPair, c_double)
+      write(90) OuterStateHisto, SignHistoSuperstates, SignHistoStates, TraceContribSuperstates, &
+                TraceContribStates
+      write(90) Gtau_mean_step, Gtau_mid_step, sign_step
+      write(90) rho1, rho2
+      close(90)
+      call write_results(d)
+   end subroutine run_calib
+
+   subroutine sort_real(a)
+      real(c_double), intent(inout) :: a(:)
+      integer :: i, j
+      real(c_double) :: x
+      do i = 2, size(a)
+         x = a(i)
+         j = i - 1
+         do while (j >= 1)
+            if (a(j) <= x) exit
+            a(j + 1) = a(j)
+            j = j - 1
+         end do
+         a(j + 1) = x
+      end do
+   end subroutine sort_real
 
    subroutine start_from_config(d)
       character(*), intent(in) :: d

@@ -17,7 +17,7 @@ program host_run
    character(len=16) :: mode
    integer :: nb, nft, seed, ios, nch
    integer(c_int64_t) :: nwarm, nmoves
-   real(c_double) :: taudiffmax
+   real(c_double) :: taudiffmax, tdm
    real(c_double), allocatable :: umat(:,:,:,:), ftau(:,:,:,:,:), muimp(:,:,:,:), screen(:,:,:,:,:)
    character(len=:), allocatable :: pstr
    character(len=600) :: line
@@ -34,7 +34,7 @@ program host_run
    read (10, *, iostat=ios) nch
    if (ios == 0) n_chains = nch
    close (10)
-   if (trim(mode) /= 'run') stop 'host_run: only mode run'
+   if (trim(mode) /= 'run' .and. trim(mode) /= 'calib') stop 'host_run: modes run and calib'
 
    pstr = ''
    open (11, file=trim(dir)//'/params.txt', status='old', action='read')
@@ -57,9 +57,37 @@ program host_run
    call init_solver(umat, ftau, muimp, screen, 1, nft, nb)
    if (aborted /= 0) stop 3
    if (taudiffmax > 0) call set_taudiffmax(taudiffmax)
+   if (trim(mode) == 'calib') then
+      ! cthyb_solver.py:899-944 up to the phase-2 histogram (the driver's fit
+      ! of it is left out: the run keeps the phase-1 TaudiffMax)
+      call init_counters()
+      call ctqmc_calibrate(.true., 1000, 100, 1000, 100000)
+      if (aborted /= 0) stop 5
+      tdm = AccPairTau(int(0.995d0*1000) + 1)
+      call set_taudiffmax(tdm)
+      call init_counters()
+      call ctqmc_calibrate(.false., 1000, 100, 1000, 100000)
+      if (aborted /= 0) stop 6
+      open (71, file=trim(dir)//'/calib.bin', access='stream', form='unformatted', status='replace', &
+            action='write')
+      write (71) int(size(AccPairTau), c_int32_t)
+      write (71) AccPairTau
+      write (71) tdm
+      write (71) int(size(AccPair, 3), c_int32_t)
+      write (71) AccPair
+   end if
    call ctqmc_warmup(nwarm)
    call ctqmc_measure(1, 1)
    if (aborted /= 0) stop 4
+   if (trim(mode) == 'calib') then
+      write (71) int(size(AccPair, 3), c_int32_t)
+      write (71) AccPair
+      write (71) OuterStateHisto, SignHistoSuperstates, SignHistoStates, TraceContribSuperstates, &
+                 TraceContribStates
+      write (71) Gtau_mean_step, Gtau_mid_step, sign_step
+      write (71) rho1, rho2
+      close (71)
+   end if
 
    open (70, file=trim(dir)//'/results.bin', access='stream', form='unformatted', status='replace', &
          action='write')

@@ -213,6 +213,54 @@ def case_refstat(name: str, prob: P.Problem, seeds, nwarm: int):
     print(name, "runs", len(seeds), "n_up", out["mean_occ"][0, 0, 0, 0], "+-", out["err_occ"][0, 0, 0, 0])
 
 
+def case_calib(name: str, prob: P.Problem, seed: int, nwarm: int):
+    """The calibration of cthyb_solver.py:899-944 on one rank, then a run: the
+    phase-1 distance list, the TaudiffMax it gives, the phase-2 AccPair, and
+    after the measurement AccPair, the outer-state histograms, the
+    step-resolved arrays and the observables (all as the reference reports
+    them, i.e. normalised)."""
+    q = prob.qmc()
+    nb, ntau, nleg, nmeas = prob.nbands, int(q["Ntau"]), int(q["NLegMax"]), int(q["Nmeas"])
+    with tempfile.TemporaryDirectory() as d:
+        write_inputs(d, prob, "calib", seed, nwarm, 0)
+        run_driver(d)
+        model = read_model(d, nb)
+        f = open(os.path.join(d, "calib.bin"), "rb")
+
+        def take(dt, n):
+            return np.fromfile(f, dtype=dt, count=n)
+        n1 = int(take("<i4", 1)[0])
+        out = {"calib_accpairtau": take("<f8", n1), "calib_taudiffmax": float(take("<f8", 1)[0])}
+        n2 = int(take("<i4", 1)[0])
+        out["calib_accpair"] = take("<f8", nb * 2 * n2).reshape((nb, 2, n2), order="F")
+        n3 = int(take("<i4", 1)[0])
+        out["res_accpair"] = take("<f8", nb * 2 * n3).reshape((nb, 2, n3), order="F")
+        nst, nss = 4 ** nb, model["nsst"]
+        for key, n in (("outer_state", nst), ("sign_sst", nss), ("sign_state", nst), ("contrib_sst", nss),
+                       ("contrib_state", nst)):
+            out["res_" + key] = take("<f8", n)
+        out["res_gtau_mean_step"] = take("<f8", nb * 2 * nmeas).reshape((nb, 2, nmeas), order="F")
+        out["res_gtau_mid_step"] = take("<f8", nb * 2 * nmeas).reshape((nb, 2, nmeas), order="F")
+        out["res_sign_step"] = take("<f8", nmeas)
+        n2 = 2 * nb
+        out["res_rho1"] = take("<f8", n2 ** 2).reshape((n2, n2), order="F")
+        out["res_rho2"] = take("<f8", n2 ** 4).reshape((n2,) * 4, order="F")
+        out["res_densitymatrix"] = None
+        f.close()
+        raw = np.fromfile(os.path.join(d, "results.bin"), dtype="<f8")
+    out["res_mean_sign"] = raw[0]
+    out["res_gtau"] = raw[1:1 + nb * 2 * ntau].reshape((nb, 2, ntau), order="F")
+    off = 1 + nb * 2 * ntau + nb * 2 * nleg + (2 * nb) ** 2
+    nst = 4 ** nb
+    out["res_densitymatrix"] = raw[off:off + nst * nst].reshape((nst, nst), order="F")
+    out.update(problem_arrays(prob))
+    out.update({"seed": seed, "nwarm": nwarm, "nmeas": nmeas})
+    out.update({"model_" + k: v for k, v in model.items()})
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
+    print(name, "taudiffmax", out["calib_taudiffmax"], "list", n1, "phase-2 pairs",
+          out["calib_accpair"].sum((0, 2)) if nb else 0)
+
+
 CASES = {
     # single window (TaudiffMax <= 0), 1 orbital, scalar trace path
     "c1_log_w1": lambda: case_log("c1_log_w1", P.anderson_1orb(), 43890, 20000),
@@ -237,6 +285,10 @@ CASES = {
                                          777, 10000, every=1),
     "c4_log_b40": lambda: case_log_long("c4_log_b40", P.coulomb_5orb(beta=40.0, TaudiffMax=5.0),
                                         2048, 1500, every=1),
+    # calibration of TaudiffMax (TaudiffMax = -1), then a run with the histograms and step arrays
+    "c2_calib": lambda: case_calib("c2_calib", P.kanamori_2orb(
+        beta=10.0, TaudiffMax=-1.0, Nmeas=200, NCorr=20, Ntau=50, NLegMax=10, Gtau_mean_step=1,
+        Gtau_mid_step=1, sign_step=1), 321, 2000),
     # 16 independent reference runs for the statistical comparison of batched chains
     "c2_refstat": lambda: case_refstat("c2_refstat", P.kanamori_2orb(
         beta=10.0, Nmeas=20000, NCorr=40, Ntau=50, NLegMax=20, TaudiffMax=3.0), list(range(500, 516)), 5000),

@@ -151,3 +151,34 @@ def test_pooled_estimate_survives_a_zero_sign_chain():
     r = ctqmc.pooled_estimate(s, nb, beta, ntau, nleg, MODEL1)
     for k in ("gtau", "gleg", "densitymatrix", "occ", "mean_sign"):
         assert np.all(np.isfinite(r[k])) and np.all(np.isfinite(r[k + "_err"])), k
+
+
+def test_reduced_density_matrices_match_reference():
+    """rho1 and rho2 from the reference's occupation-basis density matrix agree
+    with the reference's own Meas_rho1/rho2_from_Densitymatrix (CTQMC.F90:2936-3060)."""
+    from conftest import load_golden
+    z = load_golden("c2_calib")
+    r1, r2 = ctqmc.reduced_density_matrices(np.asarray(z["res_densitymatrix"]), int(z["nbands"]),
+                                            np.asarray(z["model_states"]))
+    assert np.allclose(r1, z["res_rho1"], rtol=1e-10, atol=1e-13)
+    assert np.allclose(r2, z["res_rho2"], rtol=1e-10, atol=1e-13)
+
+
+def test_accumulation_guard_catches_a_lost_bin():
+    """The G(tau) bin sum equals the l=0 Legendre sum on consistent
+    accumulators (here the oracle's) and the guard raises when one bin's
+    contribution is lost."""
+    import oracle as O
+    prob = P.kanamori_2orb(beta=10.0, TaudiffMax=3.0, Ntau=40, NLegMax=8)
+    o = O.OracleChain.from_problem(prob, 3)
+    o.set_taudiffmax(3.0)
+    o.warmup(2000)
+    o.measure(50, 20)
+    r = o.raw()
+    acc = {"gtau": r["gtau"][None], "gleg": r["gleg"][None]}
+    ctqmc.check_accumulators(acc)
+    i = int(np.argmax(np.abs(r["gtau"][0])))
+    acc["gtau"] = acc["gtau"].copy()
+    acc["gtau"][0, 0, i] = 0.0              # one bin's atomic adds lost
+    with pytest.raises(ctqmc.AccumulationError):
+        ctqmc.check_accumulators(acc)

@@ -15,7 +15,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from conftest import ROOT, load_golden
+from conftest import ROOT, load_golden, qmc_of
 
 DRIVER = os.path.join(ROOT, "tests", "fortran", "host_run")
 FLIB = os.path.join(ROOT, "w2dynamics_amd", "libw2dyn_ctqmc_fortran.so")
@@ -25,14 +25,14 @@ ENTRY_POINTS = ["init_paras", "init_rnd", "init_solver", "set_taudiffmax", "ctqm
                 "set_mc_config", "set_empty_mc_config", "dest_ctqmc", "dest_paras"]
 
 
-def write_inputs_from_fixture(d, z, nchains=1):
+def write_inputs_from_fixture(d, z, nchains=1, mode="run"):
     """Input directory of the reference driver for a golden run fixture."""
     nb = int(z["nbands"])
     ftau = np.asarray(z["ftau"])
     with open(os.path.join(d, "run.txt"), "w") as f:
-        f.write("run\n%d\n%d\n%d\n%d\n0\n%.17g\n%d\n" % (nb, ftau.shape[-1], int(z["seed"]),
-                                                       int(z["nwarm"]),
-                                                       float(z["taudiff_max_param"]), nchains))
+        f.write("%s\n%d\n%d\n%d\n%d\n0\n%.17g\n%d\n" % (mode, nb, ftau.shape[-1], int(z["seed"]),
+                                                         int(z["nwarm"]),
+                                                         float(z["taudiff_max_param"]), nchains))
     with open(os.path.join(d, "params.txt"), "w") as f:
         f.write(str(z["param_string"]))
     n = 2 * nb
@@ -44,7 +44,7 @@ def write_inputs_from_fixture(d, z, nchains=1):
 def read_results(d, z):
     nb, ntau, nleg = int(z["nbands"]), int(z["ntau"]), int(z["nlegmax"])
     nst = 4 ** nb
-    maxh = np.asarray(z["res_histo"]).shape[-1] - 1
+    maxh = int(qmc_of(z)["MaxHisto"])
     nsst = int(z["model_nsst"])
     raw = np.fromfile(os.path.join(d, "results.bin"), dtype="<f8")
     out, off = {}, 0
@@ -119,3 +119,57 @@ def test_fortran_host_batched_chains_pooled(tmp_path):
     for k in ("gtau", "gleg", "occ", "histo"):
         assert np.allclose(res[k], py[k], rtol=1e-10, atol=1e-12), k
     assert np.isclose(res["mean_sign"], py["mean_sign"], rtol=1e-12)
+
+
+@pytest.mark.gpu
+def test_fortran_host_calibration_flow_matches_reference(tmp_path):
+    """The MCTQMC module called as cthyb_solver.py does with TaudiffMax = -1:
+    init_counters, ctqmc_calibrate (list), TaudiffMax from the 99.5 % point of
+    AccPairTau, set_taudiffmax, init_counters, ctqmc_calibrate (histogram),
+    warm-up, measurement; then the arrays get_resultset reads that this run
+    fills (AccPair, outer-state histograms, step arrays, rho1, rho2, G(tau))
+    against the reference's own run of the same flow."""
+    import sys
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from test_oracle import _blocks_of
+    z = load_golden("c2_calib")
+    write_inputs_from_fixture(str(tmp_path), z, mode="calib")
+    print("fortran host calibration flow", flush=True)
+    r = subprocess.run([DRIVER, str(tmp_path)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    nb, nmeas = int(z["nbands"]), int(z["nmeas"])
+    nst, nss = 4 ** nb, int(z["model_nsst"])
+    with open(os.path.join(str(tmp_path), "calib.bin"), "rb") as f:
+        def take(dt, n):
+            return np.fromfile(f, dtype=dt, count=n)
+        n1 = int(take("<i4", 1)[0])
+        lst = take("<f8", n1)
+        tdm = float(take("<f8", 1)[0])
+        n2 = int(take("<i4", 1)[0])
+        ap2 = take("<f8", nb * 2 * n2).reshape((nb, 2, n2), order="F")
+        n3 = int(take("<i4", 1)[0])
+        ap3 = take("<f8", nb * 2 * n3).reshape((nb, 2, n3), order="F")
+        hist = {k: take("<f8", n) for k, n in (("outer_state", nst), ("sign_sst", nss), ("sign_state", nst),
+                                              ("contrib_sst", nss), ("contrib_state", nst))}
+        ms = take("<f8", nb * 2 * nmeas).reshape((nb, 2, nmeas), order="F")
+        ds = take("<f8", nb * 2 * nmeas).reshape((nb, 2, nmeas), order="F")
+        ss = take("<f8", nmeas)
+        rho1 = take("<f8", (2 * nb) ** 2).reshape((2 * nb,) * 2, order="F")
+        rho2 = take("<f8", (2 * nb) ** 4).reshape((2 * nb,) * 4, order="F")
+    # the module exposes the sorted list (the driver sorts it anyway)
+    assert np.allclose(lst, np.sort(z["calib_accpairtau"]), rtol=1e-12, atol=1e-12)
+    assert tdm == float(z["calib_taudiffmax"])
+    assert np.array_equal(ap2, z["calib_accpair"]) and np.array_equal(ap3, z["res_accpair"])
+    for k in ("sign_sst", "contrib_sst"):
+        assert np.allclose(hist[k], z["res_" + k], rtol=1e-10, atol=1e-14, equal_nan=True), k
+    for k in ("sign_state", "contrib_state", "outer_state"):
+        for a, b in _blocks_of(z):
+            assert np.allclose(np.sort(hist[k][a:b]), np.sort(z["res_" + k][a:b]), rtol=1e-10, atol=1e-14,
+                               equal_nan=True), (k, a, b)
+    assert np.allclose(ms, z["res_gtau_mean_step"], rtol=1e-10, atol=1e-14)
+    assert np.allclose(ds, z["res_gtau_mid_step"], rtol=1e-10, atol=1e-14)
+    assert np.array_equal(ss, z["res_sign_step"])
+    assert np.allclose(rho1, z["res_rho1"], rtol=1e-9, atol=1e-12)
+    assert np.allclose(rho2, z["res_rho2"], rtol=1e-9, atol=1e-12)
+    res = read_results(str(tmp_path), z)
+    assert np.allclose(res["gtau"], z["res_gtau"], rtol=1e-10, atol=1e-13)

@@ -319,3 +319,60 @@ def test_binning_accumulator_matches_oracle():
     assert np.allclose(a[fin], b[fin], rtol=1e-9, atol=1e-13)
     assert np.allclose(res["gtau"][0], ora.results()["gtau"], rtol=1e-9, atol=1e-12)
     gpu.close()
+
+
+def test_calibration_and_histograms_match_reference():
+    """ctqmc_calibrate phases 1 and 2 and the run after them on the GPU (chain
+    0): the distance list, TaudiffMax, both AccPair histograms, the outer-state
+    histograms and the step-resolved arrays, against the reference."""
+    from test_oracle import _blocks_of, _calib_flow
+    from w2dynamics_amd.ctqmc import normalise_histograms
+    z = load_golden("c2_calib")
+    q = qmc_of(z)
+    gpu = CtqmcGpu(str(z["param_string"]), z["umatrix"], z["ftau"], z["muimp"], nchains=2, seed=int(z["seed"]))
+    print("calibration flow", flush=True)
+    lst, tdm, ap2 = _calib_flow(gpu, z, q)
+    assert np.allclose(lst, z["calib_accpairtau"], rtol=1e-12, atol=1e-12)
+    assert tdm == float(z["calib_taudiffmax"])
+    nb = int(z["nbands"])
+    assert np.array_equal(ap2.reshape(2, nb, -1).swapaxes(0, 1), z["calib_accpair"])
+    gpu.warmup(int(z["nwarm"]))
+    gpu.measure(int(z["nmeas"]))
+    acc = gpu.accumulators()
+    hist = gpu.group_histograms(2)            # one chain per group
+    st = gpu.steps(0)
+    a0 = {k: v[:1] for k, v in hist.items() if k != "accpair"}
+    a0.update(outer_hist=acc["outer_hist"][:1], accpair=gpu.pair_stats(0)["accpair"][None],
+              gtau_mean_step=st["gtau_mean_step"][None], gtau_mid_step=st["gtau_mid_step"][None],
+              sign_step=st["sign_step"][None])
+    h = normalise_histograms(a0, acc["scalars"][:1, 1], nb, float(z["beta"]))
+    assert np.array_equal(h["accpair"][0], z["res_accpair"])
+    for k in ("sign_sst", "contrib_sst"):
+        assert np.allclose(h[k][0], z["res_" + k], rtol=1e-10, atol=1e-14, equal_nan=True), k
+    for k, ref in (("sign_state", "res_sign_state"), ("contrib_state", "res_contrib_state"),
+                   ("outer_state_histo", "res_outer_state")):
+        for a, b in _blocks_of(z):
+            assert np.allclose(np.sort(h[k][0][a:b]), np.sort(z[ref][a:b]), rtol=1e-10, atol=1e-14,
+                               equal_nan=True), (k, a, b)
+    for k in ("gtau_mean_step", "gtau_mid_step"):
+        assert np.allclose(h[k][0], z["res_" + k], rtol=1e-10, atol=1e-14), k
+    assert np.array_equal(h["sign_step"][0], z["res_sign_step"])
+    assert np.allclose(gpu.results()["gtau"][0], z["res_gtau"], rtol=1e-10, atol=1e-13)
+    gpu.close()
+
+
+def test_set_taudiffmax_after_moves_matches_oracle():
+    """A new window count after the chains have moved: the chains move into a
+    new layout (boundary caches rebuilt) and the decisions continue exactly as
+    the oracle's."""
+    prob = P.kanamori_2orb(beta=20.0, TaudiffMax=-1.0)
+    seed = 515
+    gpu = CtqmcGpu.from_problem(prob, nchains=2, seed=seed)
+    ora = OracleChain.from_problem(prob, seed)
+    print("set_taudiffmax after moves", flush=True)
+    _compare(gpu.moves_log(0, 3000), ora.moves(3000), "one window")
+    for tdm in (4.0, 2.0, 7.0):        # 3, 8 and 1 windows
+        gpu.set_taudiffmax(tdm)
+        ora.set_taudiffmax(tdm)
+        _compare(gpu.moves_log(0, 3000), ora.moves(3000), "TaudiffMax %g" % tdm)
+    gpu.close()

@@ -75,7 +75,7 @@ def test_move_log_matches_reference_golden(name):
     gpu = CtqmcGpu(str(z["param_string"]), z["umatrix"], z["ftau"], z["muimp"], nchains=1,
                    seed=int(z["seed"]))
     gpu.set_taudiffmax(float(q["TaudiffMax"]))
-    n = min(int(z["nmoves"]), 5000)
+    n = int(z["nmoves"])          # the whole reference log
     rec = gpu.moves_log(0, n)
     ref = {f: z["move_" + f][:n] for f in FIELDS + ("trace_log", "det_log", "trace_sign",
                                                     "det_sign", "cfgsign")}
@@ -105,6 +105,8 @@ def test_measurement_run_matches_oracle():
     gpu.set_taudiffmax(3.0)
     gpu.warmup(3000)
     gpu.measure(200)
+    from w2dynamics_amd.ctqmc import check_accumulators
+    check_accumulators(gpu.accumulators())
     res = gpu.results()
     for c in range(2):
         ora = OracleChain.from_problem(prob, seed + c)

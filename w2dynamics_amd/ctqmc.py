@@ -379,7 +379,12 @@ class CtqmcGpu:
         ok = int((self.chain_status()["status"] == 0).sum())
         ng = max(1, min(int(ngroups), ok))
         sums = self.group_sums(ng)
+        hist = self.group_histograms(ng)
+        nacc = self.pair_stats(0)["accpair"].shape[1]
+        hist["accpair"] = hist["accpair"][:, :, :nacc]
+        sums.update(hist)
         sums = {k: v[sums["nused"] > 0] for k, v in sums.items()}
+        check_accumulators(sums)
         return pooled_estimate(sums, self.nbands, self.beta, self.ntau, self.nleg, self.model())
 
 
@@ -390,13 +395,36 @@ def _param_value(params: str, key: str) -> float:
     raise KeyError(key)
 
 
+class AccumulationError(RuntimeError):
+    pass
+
+
+def check_accumulators(acc: dict, rtol: float = 1e-9):
+    """Guard against lost accumulations: every (annihilator, creator) pair adds
+    the same M * cfgsign to one G(tau) bin and, with P_0 = 1, to the l = 0
+    Legendre sum (CTQMC.F90:2360-2393), so for every chain (or group) and
+    flavour the raw bin sum equals the raw l = 0 sum up to rounding.  Raises
+    AccumulationError otherwise."""
+    g = np.asarray(acc["gtau"], dtype=float).sum(-1)
+    l0 = np.asarray(acc["gleg"], dtype=float)[..., 0]
+    scale = np.abs(acc["gtau"]).sum(-1) + np.abs(l0) + 1e-300
+    bad = np.abs(g - l0) > rtol * scale
+    if bad.any():
+        idx = np.argwhere(bad)[0]
+        raise AccumulationError("G(tau) bins and l=0 Legendre sum differ at %s: %r vs %r" % (
+            tuple(int(i) for i in idx), float(g[tuple(idx)]), float(l0[tuple(idx)])))
+
+
 def pooled_estimate(sums: dict, nb: int, beta: float, ntau: int, nleg: int, model: dict) -> dict:
     """Normalised observables of the pooled sums over the leading (group) axis
     of ``sums`` and their jackknife errors over that axis."""
-    keys = ("gtau", "gleg", "dm", "histo", "outer_hist", "scalars")
+    keys = [k for k in ("gtau", "gleg", "dm", "histo", "outer_hist", "scalars", "outer_state", "sign_sst",
+                        "sign_state", "contrib_sst", "contrib_state", "accpair") if k in sums]
     tot = {k: np.asarray(sums[k], dtype=float).sum(0)[None] for k in keys}
     est = normalise(tot, nb, beta, ntau, nleg, model)
     out = {k: np.asarray(v)[0] for k, v in est.items()}
+    if "densitymatrix" in out:
+        out["rho1"], out["rho2"] = reduced_density_matrices(out["densitymatrix"], nb, model["states"])
     n = len(sums["scalars"])
     if n < 2:
         for k in est:
@@ -446,6 +474,92 @@ def _normalise(acc, nb, beta, ntau, nleg, model):
     occ = np.einsum("ck,ijk,lmk->cijlm", diag, nocc, nocc)
     accr = np.stack([sc[:, 2] / sc[:, 3], sc[:, 4] / sc[:, 5], sc[:, 6] / sc[:, 7],
                      sc[:, 8] / sc[:, 9]], axis=1)
-    return dict(mean_sign=mean_sign, gtau=gtau, gleg=gleg, occ=occ, densitymatrix=dm_occ,
-                histo=fl2(acc["histo"]) / cnt[:, None, None, None], acc=accr,
-                outer_sst_histo=acc["outer_hist"] / cnt[:, None])
+    out = dict(mean_sign=mean_sign, gtau=gtau, gleg=gleg, occ=occ, densitymatrix=dm_occ,
+               histo=fl2(acc["histo"]) / cnt[:, None, None, None], acc=accr,
+               outer_sst_histo=acc["outer_hist"] / cnt[:, None])
+    out.update(normalise_histograms(acc, cnt, nb, beta))
+    return out
+
+
+def _apply_string(nb: int, ops):
+    """A product of fermionic operators (applied right to left; (f, dagger),
+    f = 2 b + s) on every occupation-basis state: target states, signs and
+    the states it does not annihilate.  Bit b + N s holds orbital b, spin s;
+    the sign counts the occupied lower bits (Operator.F90:160-172)."""
+    st = np.arange(1 << (2 * nb))
+    sign = np.ones(st.shape)
+    ok = np.ones(st.shape, bool)
+    for f, dag in reversed(ops):
+        bit = f // 2 + nb * (f % 2)
+        occ = (st >> bit) & 1
+        ok &= occ != dag
+        low = st & ((1 << bit) - 1)
+        par = np.zeros(st.shape, np.int64)
+        for k in range(bit):
+            par += (low >> k) & 1
+        sign = np.where(par % 2 == 1, -sign, sign)
+        st = st ^ (1 << bit)
+    return st, sign, ok
+
+
+def reduced_density_matrices(dm_occ: np.ndarray, nb: int, states):
+    """rho1[f2, f1] = <c+_f2 c_f1> and rho2[f4, f3, f2, f1] =
+    <c+_f4 c+_f3 c_f2 c_f1> (f = 2 b + s) from the occupation-basis density
+    matrix, as Meas_rho1/rho2_from_Densitymatrix (CTQMC.F90:2936-3060) give
+    them: Tr[rho O] = sum_i rho[i, O(i)] sign_i.  Index i of dm_occ is the
+    Fock state states[i] (OccbasisMapping)."""
+    nf = 2 * nb
+    st = np.asarray(states)
+    dm = np.zeros((1 << nf, 1 << nf))
+    dm[np.ix_(st, st)] = dm_occ
+    dm_occ = dm
+    idx = np.arange(1 << nf)
+    rho1 = np.zeros((nf, nf))
+    for f2 in range(nf):
+        for f1 in range(nf):
+            t, sg, ok = _apply_string(nb, [(f2, 1), (f1, 0)])
+            rho1[f2, f1] = np.sum(dm_occ[idx[ok], t[ok]] * sg[ok])
+    ann = {(f2, f1): _apply_string(nb, [(f2, 0), (f1, 0)]) for f2 in range(nf) for f1 in range(nf)}
+    cre = {(f4, f3): _apply_string(nb, [(f4, 1), (f3, 1)]) for f4 in range(nf) for f3 in range(nf)}
+    rho2 = np.zeros((nf,) * 4)
+    for (f2, f1), (ta, sa, oka) in ann.items():
+        ia = idx[oka]
+        if not len(ia):
+            continue
+        mid, sm = ta[oka], sa[oka]
+        for (f4, f3), (tb, sb, okb) in cre.items():
+            keep = okb[mid]
+            if keep.any():
+                rho2[f4, f3, f2, f1] = np.sum(dm_occ[ia[keep], tb[mid[keep]]] * sm[keep] * sb[mid[keep]])
+    return rho1, rho2
+
+
+def normalise_histograms(acc: dict, cnt, nb: int, beta: float) -> dict:
+    """Post-processing of the outer-state histograms, AccPair and the
+    step-resolved arrays (CTQMC.F90:8741-8764) for those present in ``acc``:
+    signs per visit, visits and trace contributions per measurement, the step
+    sums divided by beta^2; AccPair stays a count."""
+    with np.errstate(invalid="ignore", divide="ignore"):
+        return _normalise_histograms(acc, cnt, nb, beta)
+
+
+def _normalise_histograms(acc, cnt, nb, beta):
+    out = {}
+    c = np.asarray(cnt, dtype=float)[:, None]
+    nc = c.shape[0]
+    if "outer_state" in acc:
+        out["sign_sst"] = acc["sign_sst"] / acc["outer_hist"]
+        out["sign_state"] = acc["sign_state"] / acc["outer_state"]
+        out["outer_state_histo"] = acc["outer_state"] / c
+        out["contrib_sst"] = acc["contrib_sst"] / c
+        out["contrib_state"] = acc["contrib_state"] / c
+    if "accpair" in acc:
+        a = np.asarray(acc["accpair"])
+        out["accpair"] = a.reshape((nc, 2, nb) + a.shape[2:]).swapaxes(1, 2)
+    for k in ("gtau_mean_step", "gtau_mid_step"):
+        if k in acc:
+            a = np.asarray(acc[k]) / beta ** 2                  # [chain, step, f]
+            out[k] = a.reshape(a.shape[:2] + (2, nb)).transpose(0, 3, 2, 1)   # [chain, b, s, step]
+    if "sign_step" in acc:
+        out["sign_step"] = np.asarray(acc["sign_step"])
+    return out

@@ -52,6 +52,31 @@ module MCTQMC
    integer, allocatable :: OccbasisMapping(:)        ! occupation-basis state of each index
    real(KINDR) :: AccAdd(1) = 0, AccRem(1) = 0, AccGlob = 0, AccShift = 0
    integer(c_int64_t) :: TryAdd(1) = 0, TryRem(1) = 0, TryGlob = 0, TryShift = 0
+   !> four-operator and flavour-change moves are not taken by this core: zero
+   real(KINDR) :: AccAdd4 = 0, AccRem4 = 0, AccFlavc = 0
+   !> accepted-pair statistics (CTQMC.F90:66-70): AccPair is the mean over
+   !! chains of the counts per (orbital, spin, distance bin); AccPairTau the
+   !! mean over chains of each chain's sorted phase-1 distance list, so that
+   !! its 99.5 % point is the mean of the chains' 99.5 % points, the value the
+   !! driver forms over ranks (cthyb_solver.py:911-915)
+   real(KINDR), allocatable :: AccPair(:,:,:)
+   real(KINDR), allocatable :: AccPairTau(:)
+   !> MeasOuterHistograms outputs (CTQMC.F90:3285-3339, normalised as 8757-8764)
+   real(KINDR), allocatable :: OuterStateHisto(:), SignHistoSuperstates(:), SignHistoStates(:)
+   real(KINDR), allocatable :: TraceContribSuperstates(:), TraceContribStates(:)
+   !> reduced density matrices (CTQMC.F90:2936-3060), flavour index 2*(b-1)+s
+   real(KINDR), allocatable :: rho1(:,:), rho2(:,:,:,:)
+   !> off-diagonal G(tau) / Legendre G and the Lanczos histograms: paths this
+   !! core does not take, zero as in the reference for diagonal hybridisation
+   real(KINDR), allocatable :: Gtau_full(:,:,:,:,:), GLeg_full(:,:,:,:,:)
+   real(KINDR) :: lhisto(100) = 0, rhisto(100) = 0
+   !> step-resolved arrays (Gtau_mean_step, Gtau_mid_step, sign_step;
+   !! CTQMC.F90:2394-2402, 8741-8742), mean over chains; allocated if enabled
+   real(KINDR), allocatable :: Gtau_mean_step(:,:,:), Gtau_mid_step(:,:,:), sign_step(:)
+   !> AccumGtau = 2: binning variances of the raw G(tau) per level
+   !! (gtau_blocks, CTQMC.F90:8733-8739), mean over chains
+   real(KINDR), allocatable :: gtau_blocks(:,:,:,:)
+   real(KINDR) :: time_calibration = 0
 
    type(c_ptr), private, save :: handle = c_null_ptr
    character(len=:), allocatable, private, save :: paramstring
@@ -140,6 +165,45 @@ module MCTQMC
       end function
       integer(c_int) function w2d_ctqmc_trap_signals() bind(C, name="w2d_ctqmc_trap_signals")
          import :: c_int
+      end function
+      integer(c_int) function w2d_ctqmc_init_counters(h) bind(C, name="w2d_ctqmc_init_counters")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: h
+      end function
+      integer(c_int) function w2d_ctqmc_calibrate(h, list, p1, p2grid, p2pairs, nprog) &
+            bind(C, name="w2d_ctqmc_calibrate")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: h
+         integer(c_int), value :: list, p1, p2grid, p2pairs, nprog
+      end function
+      integer(c_int) function w2d_ctqmc_get_pair_stats(h, chain, apt, napt, ap, nap) &
+            bind(C, name="w2d_ctqmc_get_pair_stats")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: h
+         integer(c_int), value :: chain
+         real(c_double) :: apt(*), ap(*)
+         integer(c_int) :: napt, nap
+      end function
+      integer(c_int) function w2d_ctqmc_get_group_histograms(h, ng, ost, ssst, sst, csst, cst, ap) &
+            bind(C, name="w2d_ctqmc_get_group_histograms")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: h
+         integer(c_int), value :: ng
+         real(c_double) :: ost(*), ssst(*), sst(*), csst(*), cst(*), ap(*)
+      end function
+      integer(c_int) function w2d_ctqmc_get_steps(h, chain, nm, ms, ds, ss) bind(C, name="w2d_ctqmc_get_steps")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: h
+         integer(c_int), value :: chain
+         integer(c_int) :: nm
+         real(c_double) :: ms(*), ds(*), ss(*)
+      end function
+      integer(c_int) function w2d_ctqmc_get_gtau_blocks(h, chain, nl, blocks) bind(C, name="w2d_ctqmc_get_gtau_blocks")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: h
+         integer(c_int), value :: chain
+         integer(c_int) :: nl
+         real(c_double) :: blocks(*)
       end function
       integer(c_int) function w2d_ctqmc_destroy(h) bind(C, name="w2d_ctqmc_destroy")
          import :: c_int, c_ptr
@@ -282,10 +346,77 @@ contains
       if (failed(w2d_ctqmc_set_taudiffmax(handle, taudiffmax), "w2d_ctqmc_set_taudiffmax")) return
    end subroutine set_taudiffmax
 
-   !> init_counters (CTQMC.F90:847): the library clears its accumulators at the
-   !! start of every measurement call
+   !> init_counters (CTQMC.F90:847-899): move counters, AccPair, calibration list index
    subroutine init_counters()
+      if (failed(w2d_ctqmc_init_counters(handle), "w2d_ctqmc_init_counters")) return
    end subroutine init_counters
+
+   !> ctqmc_calibrate (CTQMC.F90:8088-8178) on every chain; fills AccPairTau
+   !! (list) or AccPair (phase 2) as described at their declarations
+   subroutine ctqmc_calibrate(list, phase1pairnum, phase2taugrid, phase2pairnum, nprogress)
+      logical, intent(in) :: list
+      integer, intent(in) :: phase1pairnum, phase2taugrid, phase2pairnum, nprogress
+      integer(8) :: t0, t1, rate
+      integer :: c
+      integer(c_int) :: napt, nap, lflag
+      real(KINDR), allocatable :: lst(:), ap(:,:)
+      call system_clock(t0, rate)
+      lflag = merge(1, 0, list)
+      if (failed(w2d_ctqmc_calibrate(handle, lflag, int(phase1pairnum, c_int), int(phase2taugrid, c_int), &
+                                     int(phase2pairnum, c_int), int(nprogress, c_int)), &
+                 "w2d_ctqmc_calibrate")) return
+      call system_clock(t1)
+      time_calibration = time_calibration + real(t1 - t0, KINDR)/real(rate, KINDR)
+      if (list) then
+         if (allocated(AccPairTau)) deallocate (AccPairTau)
+         allocate (AccPairTau(phase1pairnum), lst(phase1pairnum))
+         AccPairTau = 0
+         do c = 0, n_chains - 1
+            lst = 0
+            if (failed(w2d_ctqmc_get_pair_stats(handle, int(c, c_int), lst, napt, lst, nap), &
+                       "w2d_ctqmc_get_pair_stats")) return
+            call sort_ascending(lst(1:napt))
+            AccPairTau = AccPairTau + lst/n_chains
+         end do
+      else
+         if (allocated(AccPairTau)) deallocate (AccPairTau)
+         call collect_accpair()
+      end if
+   end subroutine ctqmc_calibrate
+
+   !> AccPair: mean over chains of the counts
+   subroutine collect_accpair()
+      integer :: c
+      integer(c_int) :: napt, nap
+      real(KINDR) :: dum(1)
+      real(KINDR), allocatable :: ap(:,:)
+      if (failed(w2d_ctqmc_get_pair_stats(handle, 0_c_int, dum, napt, dum, nap), "w2d_ctqmc_get_pair_stats")) return
+      if (allocated(AccPair)) deallocate (AccPair)
+      allocate (AccPair(nb, 2, nap), ap(nap, 2*nb))
+      AccPair = 0
+      do c = 0, n_chains - 1
+         if (failed(w2d_ctqmc_get_pair_stats(handle, int(c, c_int), dum, napt, ap, nap), &
+                    "w2d_ctqmc_get_pair_stats")) return
+         AccPair(:, 1, :) = AccPair(:, 1, :) + transpose(ap(:, 1:nb))/n_chains
+         AccPair(:, 2, :) = AccPair(:, 2, :) + transpose(ap(:, nb + 1:2*nb))/n_chains
+      end do
+   end subroutine collect_accpair
+
+   subroutine sort_ascending(a)
+      real(KINDR), intent(inout) :: a(:)
+      integer :: i, j
+      real(KINDR) :: x
+      do i = 2, size(a)
+         x = a(i)
+         j = i - 1
+         do while (j >= 1)
+            if (a(j) <= x) exit
+            a(j + 1) = a(j)
+            j = j - 1
+         end do
+         a(j + 1) = x
+      end do
+   end subroutine sort_ascending
 
    !> ctqmc_warmup (CTQMC.F90:8188): Nwarmups steps of every chain
    subroutine ctqmc_warmup(Nwarmups)
@@ -377,6 +508,10 @@ contains
       TryRem(1) = int(sc(6), c_int64_t)
       TryGlob = int(sc(8), c_int64_t)
       TryShift = int(sc(10), c_int64_t)
+      call collect_histograms(cnt, oh)
+      call collect_accpair()
+      call reduced_density_matrices()
+      call collect_steps_and_blocks()
       ! occ(b1,s1,b2,s2) = sum_k rho_kk n_{b1 s1}(k) n_{b2 s2}(k), occupation basis
       do k = 1, nstates
          diag(k) = DensityMatrix(k, k)
@@ -396,10 +531,136 @@ contains
       end do
    end subroutine collect_results
 
+   !> MeasOuterHistograms over all chains, normalised as CTQMC.F90:8757-8764
+   subroutine collect_histograms(cnt, oh_raw)
+      real(KINDR), intent(in) :: cnt, oh_raw(:)
+      real(KINDR), allocatable :: ost(:), ssst(:), sst(:), csst(:), cst(:), ap(:)
+      allocate (ost(nstates), ssst(nsst), sst(nstates), csst(nsst), cst(nstates))
+      allocate (ap(2*nb*max(ntau, 1024)))
+      if (failed(w2d_ctqmc_get_group_histograms(handle, 1_c_int, ost, ssst, sst, csst, cst, ap), &
+                 "w2d_ctqmc_get_group_histograms")) return
+      SignHistoSuperstates = ssst/oh_raw
+      SignHistoStates = sst/ost
+      OuterStateHisto = ost/cnt
+      TraceContribSuperstates = csst/cnt
+      TraceContribStates = cst/cnt
+   end subroutine collect_histograms
+
+   !> product of fermionic operators (applied right to left; flavour 2*(b-1)+s,
+   !! dagger) on an occupation-basis state: bit (b-1) + NBands*(s-1), sign
+   !! from the occupied lower bits (Operator.F90:160-172)
+   subroutine apply_ops(nop, fl, dag, st, sg, ok)
+      integer, intent(in) :: nop, fl(nop), dag(nop)
+      integer, intent(inout) :: st
+      real(KINDR), intent(out) :: sg
+      logical, intent(out) :: ok
+      integer :: i, bt
+      sg = 1
+      ok = .true.
+      do i = nop, 1, -1
+         bt = (fl(i) - 1)/2 + nb*mod(fl(i) - 1, 2)
+         if (merge(1, 0, btest(st, bt)) == dag(i)) then
+            ok = .false.
+            return
+         end if
+         if (mod(popcnt(iand(st, ishft(1, bt) - 1)), 2) == 1) sg = -sg
+         st = ieor(st, ishft(1, bt))
+      end do
+   end subroutine apply_ops
+
+   !> rho1(f2,f1) = <c+_f2 c_f1>, rho2(f4,f3,f2,f1) = <c+_f4 c+_f3 c_f2 c_f1>
+   !! (Meas_rho1/rho2_from_Densitymatrix, CTQMC.F90:2936-3060) from the
+   !! occupation-basis density matrix: Tr[rho O] = sum_k rho(k, O(k)) sign_k
+   subroutine reduced_density_matrices()
+      integer, allocatable :: idx(:)
+      integer :: nf, k, st, f1, f2, f3, f4
+      real(KINDR) :: sg
+      logical :: ok
+      nf = 2*nb
+      allocate (idx(0:2**nf - 1))
+      idx = 0
+      do k = 1, nstates
+         idx(OccbasisMapping(k)) = k
+      end do
+      if (allocated(rho1)) deallocate (rho1, rho2)
+      allocate (rho1(nf, nf), rho2(nf, nf, nf, nf))
+      rho1 = 0
+      rho2 = 0
+      do f1 = 1, nf
+         do f2 = 1, nf
+            do k = 1, nstates
+               st = OccbasisMapping(k)
+               call apply_ops(2, [f2, f1], [1, 0], st, sg, ok)
+               if (ok) rho1(f2, f1) = rho1(f2, f1) + DensityMatrix(k, idx(st))*sg
+            end do
+            do f3 = 1, nf
+               do f4 = 1, nf
+                  do k = 1, nstates
+                     st = OccbasisMapping(k)
+                     call apply_ops(4, [f4, f3, f2, f1], [1, 1, 0, 0], st, sg, ok)
+                     if (ok) rho2(f4, f3, f2, f1) = rho2(f4, f3, f2, f1) + DensityMatrix(k, idx(st))*sg
+                  end do
+               end do
+            end do
+         end do
+      end do
+   end subroutine reduced_density_matrices
+
+   !> step-resolved arrays and binning blocks (mean over chains)
+   subroutine collect_steps_and_blocks()
+      integer(c_int) :: nm, nl
+      integer :: c, b, s, f
+      real(KINDR), allocatable :: ms(:,:), ds(:,:), ss(:), bl(:,:,:)
+      real(KINDR) :: dum(1)
+      if (allocated(Gtau_mean_step)) deallocate (Gtau_mean_step)
+      if (allocated(Gtau_mid_step)) deallocate (Gtau_mid_step)
+      if (allocated(sign_step)) deallocate (sign_step)
+      if (allocated(gtau_blocks)) deallocate (gtau_blocks)
+      if (failed(w2d_ctqmc_get_steps(handle, 0_c_int, nm, dum, dum, dum), "w2d_ctqmc_get_steps")) return
+      if (nm > 0) then
+         allocate (ms(2*nb, nm), ds(2*nb, nm), ss(nm))
+         allocate (Gtau_mean_step(nb, 2, nm), Gtau_mid_step(nb, 2, nm), sign_step(nm))
+         Gtau_mean_step = 0
+         Gtau_mid_step = 0
+         sign_step = 0
+         do c = 0, n_chains - 1
+            if (failed(w2d_ctqmc_get_steps(handle, int(c, c_int), nm, ms, ds, ss), "w2d_ctqmc_get_steps")) return
+            do s = 1, 2
+               do b = 1, nb
+                  f = b + nb*(s - 1)
+                  Gtau_mean_step(b, s, :) = Gtau_mean_step(b, s, :) + ms(f, :)/beta**2/n_chains
+                  Gtau_mid_step(b, s, :) = Gtau_mid_step(b, s, :) + ds(f, :)/beta**2/n_chains
+               end do
+            end do
+            sign_step = sign_step + ss/n_chains
+         end do
+      end if
+      if (w2d_ctqmc_get_gtau_blocks(handle, 0_c_int, nl, dum) == 0) then
+         allocate (bl(ntau, 2*nb, nl), gtau_blocks(nb, 2, ntau, nl))
+         gtau_blocks = 0
+         do c = 0, n_chains - 1
+            if (failed(w2d_ctqmc_get_gtau_blocks(handle, int(c, c_int), nl, bl), "w2d_ctqmc_get_gtau_blocks")) return
+            do s = 1, 2
+               do b = 1, nb
+                  f = b + nb*(s - 1)
+                  gtau_blocks(b, s, :, :) = gtau_blocks(b, s, :, :) + bl(:, f, :)/n_chains
+               end do
+            end do
+         end do
+      end if
+   end subroutine collect_steps_and_blocks
+
    subroutine reset_results()
       if (allocated(Gtau)) deallocate (Gtau, GLeg, occ, DensityMatrix, Histo, OuterSuperstateHisto)
+      if (allocated(OuterStateHisto)) deallocate (OuterStateHisto, SignHistoSuperstates, SignHistoStates, &
+                                                  TraceContribSuperstates, TraceContribStates, Gtau_full, GLeg_full)
       allocate (Gtau(nb, 2, ntau), GLeg(nb, 2, nleg), occ(nb, 2, nb, 2), DensityMatrix(nstates, nstates))
       allocate (Histo(nb, 2, 0:maxhisto), OuterSuperstateHisto(nsst))
+      allocate (OuterStateHisto(nstates), SignHistoSuperstates(nsst), SignHistoStates(nstates))
+      allocate (TraceContribSuperstates(nsst), TraceContribStates(nstates))
+      allocate (Gtau_full(nb, 2, nb, 2, ntau), GLeg_full(nb, 2, nb, 2, nleg))
+      Gtau_full = 0
+      GLeg_full = 0
       Gtau = 0
       GLeg = 0
       occ = 0
