@@ -155,6 +155,17 @@ int w2d_ctqmc_get_accumulators(w2d_ctqmc *h, double *gtau, double *gleg, double 
 int w2d_ctqmc_get_group_sums(w2d_ctqmc *h, int ngroups, double *gtau, double *gleg, double *dm_blocks,
                              double *histo, double *outer, double *scalars, int32_t *nused);
 
+/* w2d_ctqmc_get_group_sums without any allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops).  Same arguments and results.  Its work
+ * area (ngroups x (accumulator doubles per chain + 15) x 8 bytes) lives in the
+ * library's static 16 MiB device scratch; larger requests return
+ * W2D_ERR_CAPACITY with a message on stderr.  The scratch is shared by every
+ * call on one GPU: overlapping calls on one GPU race. */
+int w2d_ctqmc_get_group_sums_noalloc(w2d_ctqmc *h, int ngroups, double *gtau, double *gleg,
+                                     double *dm_blocks, double *histo, double *outer,
+                                     double *scalars, int32_t *nused);
+
 /* AccumGtau = 2 (autocorracc, Accumulators.F90:283-362): binning analysis of
  * one chain's raw G(tau) bins, blocks[l][2N][ntau] = variance of the means of
  * bins of 2^l measurements (gtau_blocks, CTQMC.F90:8733-8739); *nlevels =

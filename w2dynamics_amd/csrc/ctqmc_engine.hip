@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -41,6 +42,7 @@ struct w2d_ctqmc {
     int *abort_dev = nullptr;
     int aborted = 0;           // of the last warmup / measure (CTQMC.F90:8648-8658)
     int regrows = 0;           // capacity doublings so far
+    void *scratch = nullptr;   // device address of the static scratch on this handle's GPU
     bool grow = true;          // capacity doubles on demand (create's capacity >= 0)
     int3 calib{0, 1, 0};       // calibration mode of the sweeps: phase, check period, pairs per flavour
     ~w2d_ctqmc();
@@ -78,6 +80,33 @@ constexpr int KCAP_MAX = 4096;   // pairs per flavour a chain may grow to
 constexpr int ACCPAIR_BINS_MAX = 1024;   // accepted-pair histogram bins (NAccPair) at most, beyond Ntau
 constexpr int APT_CAP = 65536;           // accepted-pair distances a calibration list holds at most
 
+// Static device scratch of the allocation-free entry point (16 MiB per GPU,
+// held until the process ends).  In host code `g_scratch` names a host-side
+// stand-in, not device memory: kernels get its device address, looked up once
+// per device with hipGetSymbolAddress (scratch_for_device).  Every call on one
+// GPU shares the array, so calls that overlap on one GPU race.
+__device__ __align__(256) char g_scratch[16u << 20];
+constexpr int MAX_DEVICES = 64;
+void *g_scratch_dev[MAX_DEVICES];
+std::mutex g_scratch_mu;
+
+// device address of g_scratch on the current device `dev` (looked up once)
+int scratch_for_device(int dev, void **out)
+{
+    if (dev < 0 || dev >= MAX_DEVICES) return W2D_ERR_ARG;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(g_scratch));
+        if (e != hipSuccess || !p) {
+            std::fprintf(stderr, "w2d_ctqmc: hipGetSymbolAddress(g_scratch) failed: %s\n", hipGetErrorString(e));
+            return W2D_ERR_HIP;
+        }
+        g_scratch_dev[dev] = p;
+    }
+    *out = g_scratch_dev[dev];
+    return W2D_OK;
+}
 
 // abort words of the live handles, raised together by the signal handler
 constexpr int MAX_TRAPPED = 64;
@@ -676,6 +705,10 @@ int w2d_ctqmc_create(const char *params, int nbands, int nftau, const double *u_
     *h->abort_host = 0;
     HIPCHK(hipHostGetDevicePointer(reinterpret_cast<void **>(&h->abort_dev), h->abort_host, 0));
     register_abort_word(h->abort_host);
+    {
+        const int sst = scratch_for_device(device, &h->scratch);
+        if (sst) return sst;
+    }
     // device model
     w2d::DevModel &d = h->dm;
     const int nf = 2 * nbands;
@@ -1068,6 +1101,22 @@ int w2d_ctqmc_get_group_sums(w2d_ctqmc *h, int ngroups, double *gtau, double *gl
     const int st = group_sums_into(h, ngroups, work, gtau, gleg, dm_blocks, histo, outer, scalars, nused);
     (void)hipFree(work);
     return st;
+}
+
+int w2d_ctqmc_get_group_sums_noalloc(w2d_ctqmc *h, int ngroups, double *gtau, double *gleg, double *dm_blocks,
+                                     double *histo, double *outer, double *scalars, int32_t *nused)
+{
+    if (!h || ngroups < 1 || ngroups > h->nchains) return W2D_ERR_ARG;
+    if (!h->measured) return W2D_ERR_STATE;
+    if (!h->scratch) return W2D_ERR_HIP;
+    if (group_sums_bytes(h, ngroups) > sizeof(g_scratch)) {
+        std::fprintf(stderr,
+                     "w2d_ctqmc_get_group_sums_noalloc: %d groups of this handle need %zu bytes of scratch, "
+                     "the library's static scratch holds %zu\n",
+                     ngroups, group_sums_bytes(h, ngroups), sizeof(g_scratch));
+        return W2D_ERR_CAPACITY;
+    }
+    return group_sums_into(h, ngroups, h->scratch, gtau, gleg, dm_blocks, histo, outer, scalars, nused);
 }
 
 int w2d_ctqmc_get_gtau_blocks(w2d_ctqmc *h, int chain, int *nlevels, double *blocks)

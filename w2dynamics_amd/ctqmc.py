@@ -91,6 +91,7 @@ def load_library(path: str | None = None):
     lib.w2d_ctqmc_get_pair_stats.argtypes = [P, I, P, C.POINTER(I), P, C.POINTER(I)]
     lib.w2d_ctqmc_get_group_histograms.argtypes = [P, I, P, P, P, P, P, P]
     lib.w2d_ctqmc_get_steps.argtypes = [P, I, C.POINTER(I), P, P, P]
+    lib.w2d_ctqmc_get_group_sums_noalloc.argtypes = [P, I, P, P, P, P, P, P, P]
     lib.w2d_ctqmc_version.restype = C.c_char_p
     _lib = lib
     return lib
