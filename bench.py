@@ -122,6 +122,8 @@ def thermalise(gpu, ncorr, min_sweeps, chunk, budget_s, tol=0.015):
         gpu.warmup(chunk * ncorr)
         sweeps += chunk
         traj.append((sweeps, float(gpu.orders().mean()), time.perf_counter() - t0))
+        print("thermalisation: %d sweeps, mean order %.2f, %.1f s" % traj[-1], file=sys.stderr,
+              flush=True)
         m = [x[1] for x in traj[1:]]
         if sweeps >= min_sweeps and len(m) >= 4 and \
                 abs(np.mean(m[-2:]) - np.mean(m[-4:-2])) <= tol * max(1.0, np.mean(m[-2:])):
@@ -142,6 +144,7 @@ def run_gpu(cfg, nchains, capacity, min_sweeps, steps, ncorr, seed, therm_s):
     chunk = 5 if cfg in ("c3", "c4") else 20
     traj = thermalise(gpu, ncorr, min_sweeps, chunk, therm_s)
     order_before = float(gpu.orders().mean())
+    print("timed measurement: %d sweeps" % steps, file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     gpu.measure(steps)
@@ -344,6 +347,7 @@ def main():
     capacity = WORKLOADS[cfg][4]
     seed = args.seed
 
+    print("batched run: %s, %d chains" % (cfg, nchains), file=sys.stderr, flush=True)
     b = run_gpu(cfg, nchains, capacity, args.warmup, args.steps, args.ncorr, seed, args.therm_s)
     value = b["moves"] / b["wall"]
     inv_err2_s = b["inv_err2"] / b["wall"]
@@ -370,6 +374,7 @@ def main():
 
     plain = None
     if not args.no_plain:
+        print("plain single-chain run", file=sys.stderr, flush=True)
         p = run_plain(cfg, capacity, args.ncorr, seed + 1000003, b["configs"][0])
         plain = dict(moves_per_s=p["moves"] / p["wall"], gtau_inv_err2_per_s=p["inv_err2"] / p["wall"],
                      sweeps=p["nmeas"], wall_s=p["wall"], mean_order=p["order"],
@@ -378,6 +383,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
+        print("CPU baseline", file=sys.stderr, flush=True)
         cpu = cpu_baseline(cfg, args.ncorr, seed, b["configs"])
 
     st = b["status"]

@@ -243,6 +243,39 @@ int w2d_ctqmc_aborted(const w2d_ctqmc *h, int *aborted);
  * handler raises the abort word of every live handle. */
 int w2d_ctqmc_trap_signals(void);
 
+/* ---- parameter derivatives (no counterpart in the reference) ---------------
+ * For callers that differentiate observables with respect to model parameters
+ * theta (score-function estimators, d<O>/dtheta = <O s> - <O><s> + <dO/dtheta>
+ * with s = d ln|w| / dtheta of the configuration weight
+ * w = det(Delta) Tr[...], CTQMC.F90's weight of MatrixUpdate.F90:41-99 and
+ * Trace.F90:5734-6102).
+ *
+ * Derivatives of the create inputs for ntheta (1..4) parameters, each array
+ * laid out like its create counterpart and repeated ntheta times:
+ *   du     [ntheta][(2N)^4]           d u_matrix / d theta (H must stay linear)
+ *   dmuimp [ntheta][N][2][N][2]       d muimp_full / d theta
+ *   dftau  [ntheta][N][2][N][2][Nftau] d ftau_full / d theta (diagonal read)
+ * dH/dtheta must conserve the quantum numbers (else W2D_ERR_ARG).  score != 0
+ * adds the score sums below to every later measurement; chains keep their
+ * state (their storage is re-laid out). */
+int w2d_ctqmc_set_derivatives(w2d_ctqmc *h, int ntheta, const double *du, const double *dmuimp,
+                              const double *dftau, int score);
+
+/* d ln|det Delta| / d theta_t -> out[2 t] and d ln|Tr| / d theta_t -> out[2 t + 1]
+ * of chain `chain`'s current configuration (the trace part without the Egs
+ * shift of the weights that get_config reports). */
+int w2d_ctqmc_dlogw(w2d_ctqmc *h, int chain, double *out);
+
+/* Score sums of the last measure over contiguous groups of chains (like
+ * w2d_ctqmc_get_group_sums), with s_t of each measured configuration:
+ *   ssum   [ngroups][ntheta]               sum of sign x s_t
+ *   gtau   [ngroups][ntheta][2N][ntau]     sum of s_t x the G(tau) bins of a measurement
+ *   dm     [ngroups][ntheta][dmsize]       sum of s_t x the density-matrix sample (blocks)
+ *   dmx    [ngroups][ntheta][dmsize]       sum of d(sample)/d theta_t at fixed configuration
+ * (NULL skips an array).  W2D_ERR_STATE without score sums. */
+int w2d_ctqmc_get_score_sums(w2d_ctqmc *h, int ngroups, double *ssum, double *gtau, double *dm,
+                             double *dmx);
+
 /* Replaces dest_ctqmc (CTQMC.F90:901). */
 int w2d_ctqmc_destroy(w2d_ctqmc *h);
 

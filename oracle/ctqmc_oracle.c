@@ -168,8 +168,16 @@ typedef struct {
     int accum, nlev;
     long acc_n;
     double *gmeas, *acc_spill, *acc_mean, *acc_sq;   /* [ncomp], [nlev][ncomp] x 3 */
+    /* parameter derivatives (no reference counterpart): dH/dtheta blocks in
+     * the eigenbasis [ntheta][sum n_s^2] at dboff[s], dF/dtheta
+     * [ntheta][nf][nftau]; score sums (Sum sign s, Sum G-sample s, Sum
+     * rho-sample s, Sum d(rho-sample)/dtheta) */
+    int ntheta, score;
+    long dbsize, *dboff;
+    double *dblk, *dftau;
+    double *sc_sum, *sc_g, *sc_dm, *sc_dmx;
     /* scratch */
-    double *ket, *tmp;
+    double *ket, *tmp, *dket;
     double *kets_b2, *bras_b2, *klog_b2, *blog_b2;
     int sst_b2;
 } ochain_t;
@@ -1262,6 +1270,326 @@ static void meas_misc(ochain_t *ch)
     }
 }
 
+/* ------------------------ parameter derivatives -------------------------- */
+/* d ln|w| / d theta of a configuration (no reference counterpart; pinned by
+ * finite differences of local_trace and full_inverse at theta +- delta).
+ * w = det(Delta) Tr[e^{-(beta - tau_n) H} O_n ... O_1 e^{-tau_1 H}]:
+ *   d ln|det Delta| = sum_f tr(Delta_f^{-1} dDelta_f), dDelta from dF/dtheta;
+ *   d ln|Tr| from tangent vectors dv carried beside every state vector v:
+ *   over a segment dt in block s (eigenbasis, levels E_k),
+ *   d e^{-dt H} = -(D o Phi) with D = V^T dH V and the divided difference
+ *   Phi_kl = (e^{-dt E_k} - e^{-dt E_l}) / (E_l - E_k) (dt e^{-dt E_k} if equal);
+ *   an operator block O maps v and dv alike. */
+static double phi_div(double ek, double el, double dt)
+{
+    /* levels relative to the block's lowest; evaluated from the smaller one */
+    double a = ek < el ? ek : el, y = fabs(ek - el) * dt;
+    double g = y > 0.0 ? -expm1(-y) / y : 1.0;
+    return dt * exp(-a * dt) * g;
+}
+
+/* v <- P v, dv_t <- P dv_t - (D_t o Phi) v over dt in block s (relative levels) */
+static void prop_tangent(const ochain_t *ch, int s, double dt, double *v, double *dv, int ns)
+{
+    const double *E = ch->evals + ch->offsets[s];
+    double *tmp = ch->tmp;
+    for (int t = 0; t < ch->ntheta; ++t) {
+        const double *D = ch->dblk + (long)t * ch->dbsize + ch->dboff[s];
+        double *d = dv + (long)t * ch->nmax;
+        for (int k = 0; k < ns; ++k) {
+            double acc = exp(-(E[k] - E[0]) * dt) * d[k];
+            for (int l = 0; l < ns; ++l)
+                acc -= D[k + (long)l * ns] * phi_div(E[k] - E[0], E[l] - E[0], dt) * v[l];
+            tmp[k] = acc;
+        }
+        memcpy(d, tmp, sizeof(double) * (size_t)ns);
+    }
+    for (int k = 0; k < ns; ++k) v[k] *= exp(-(E[k] - E[0]) * dt);
+}
+
+static void apply_tangent(const ochain_t *ch, const double *P, int nt, int ns, double *v, double *dv)
+{
+    double *w = ch->tmp;
+    for (int r = 0; r < nt; ++r) {
+        double acc = 0.0;
+        for (int c = 0; c < ns; ++c) acc += P[r + (long)c * nt] * v[c];
+        w[r] = acc;
+    }
+    memcpy(v, w, sizeof(double) * (size_t)nt);
+    for (int t = 0; t < ch->ntheta; ++t) {
+        double *d = dv + (long)t * ch->nmax;
+        for (int r = 0; r < nt; ++r) {
+            double acc = 0.0;
+            for (int c = 0; c < ns; ++c) acc += P[r + (long)c * nt] * d[c];
+            w[r] = acc;
+        }
+        memcpy(d, w, sizeof(double) * (size_t)nt);
+    }
+}
+
+/* rescale v and its tangents by one factor (the largest entry of any of them) */
+static void split_log_tangent(const ochain_t *ch, double *v, double *dv, int n, double *lf)
+{
+    double mv = 0.0;
+    for (int i = 0; i < n; ++i) mv = fmax(mv, fabs(v[i]));
+    for (int t = 0; t < ch->ntheta; ++t)
+        for (int i = 0; i < n; ++i) mv = fmax(mv, fabs(dv[(long)t * ch->nmax + i]));
+    if (mv > 0.0) {
+        for (int i = 0; i < n; ++i) v[i] /= mv;
+        for (int t = 0; t < ch->ntheta; ++t)
+            for (int i = 0; i < n; ++i) dv[(long)t * ch->nmax + i] /= mv;
+        *lf += log(mv);
+    }
+}
+
+/* walk of the unit vector i0 of the outer superstate from tau = 0 through every
+ * operator to beta with tangents: value, tangents and log scale of <i0| ... |i0> */
+static void walk_tangent_full(const ochain_t *ch, int i0, double *val, double *dval, double *lgo)
+{
+    int s = ch->outer_sst, ns = ch->sizes[s];
+    double *v = ch->ket, *dv = ch->dket;
+    double lg = 0.0, t0 = 0.0;
+    memset(v, 0, sizeof(double) * (size_t)ch->nmax);
+    memset(dv, 0, sizeof(double) * (size_t)ch->nmax * ch->ntheta);
+    v[i0] = 1.0;
+    for (int j = 0; j < ch->nop; ++j) {
+        const oop_t *o = &ch->op[j];
+        int t = conn(ch, s, o->orb, o->spin, o->ca), nt = ch->sizes[t];
+        const double *E = ch->evals + ch->offsets[s];
+        prop_tangent(ch, s, o->tau - t0, v, dv, ns);
+        lg += -(E[0] - ch->egs) * (o->tau - t0);
+        apply_tangent(ch, opblk(ch, s, o->orb, o->spin, o->ca), nt, ns, v, dv);
+        split_log_tangent(ch, v, dv, nt, &lg);
+        s = t, ns = nt, t0 = o->tau;
+    }
+    const double *E = ch->evals + ch->offsets[s];
+    prop_tangent(ch, s, ch->beta - t0, v, dv, ns);
+    lg += -(E[0] - ch->egs) * (ch->beta - t0);
+    *val = v[i0];
+    for (int t = 0; t < ch->ntheta; ++t) dval[t] = dv[(long)t * ch->nmax + i0];
+    *lgo = lg;
+}
+
+/* d ln|Tr| / d theta: sum over the outer states of the walks above */
+static void trace_tangent(const ochain_t *ch, double *dlt)
+{
+    int no = ch->sizes[ch->outer_sst];
+    double *val = malloc(sizeof(double) * (size_t)no), *lg = malloc(sizeof(double) * (size_t)no);
+    double *dval = malloc(sizeof(double) * (size_t)no * ch->ntheta);
+    double lmax = -DBL_MAX;
+    for (int i0 = 0; i0 < no; ++i0) {
+        walk_tangent_full(ch, i0, &val[i0], &dval[(long)i0 * ch->ntheta], &lg[i0]);
+        if (lg[i0] > lmax) lmax = lg[i0];
+    }
+    double tr = 0.0;
+    for (int i0 = 0; i0 < no; ++i0) tr += val[i0] * exp(lg[i0] - lmax);
+    for (int t = 0; t < ch->ntheta; ++t) {
+        double d = 0.0;
+        for (int i0 = 0; i0 < no; ++i0) d += dval[(long)i0 * ch->ntheta + t] * exp(lg[i0] - lmax);
+        dlt[t] = d / tr;
+    }
+    free(val), free(lg), free(dval);
+}
+
+/* d ln|det| / d theta = sum_f tr(Delta_f^{-1} dDelta_f) with a fresh LU */
+static void det_tangent(ochain_t *ch, double *dld)
+{
+    for (int t = 0; t < ch->ntheta; ++t) dld[t] = 0.0;
+    const double *ftau = ch->ftau;
+    for (int f = 0; f < NF(ch); ++f) {
+        int k = count_flavour(ch, f) / 2;
+        if (k == 0) continue;
+        double *D = malloc(sizeof(double) * (size_t)k * k), *inv = malloc(sizeof(double) * (size_t)k * k);
+        double *dD = malloc(sizeof(double) * (size_t)k * k);
+        build_delta(ch, f, D);
+        olog_t d;
+        lu_logdet(k, D, &d, inv);
+        for (int t = 0; t < ch->ntheta; ++t) {
+            ch->ftau = ch->dftau + (long)t * NF(ch) * ch->nftau;   /* dF/dtheta tables */
+            build_delta(ch, f, dD);
+            ch->ftau = ftau;
+            double acc = 0.0;
+            for (int i = 0; i < k; ++i)
+                for (int j = 0; j < k; ++j) acc += inv[i + (long)j * k] * dD[j + (long)i * k];
+            dld[t] += acc;
+        }
+        free(D), free(inv), free(dD);
+    }
+}
+
+/* ket and bra of the outer state i0 at beta/2 with tangents (as
+ * beta_half_states); returns the superstate at beta/2 */
+static int half_tangent(const ochain_t *ch, int i0, double *k, double *dk, double *kl, double *b, double *db,
+                        double *bl)
+{
+    double bh = ch->beta / 2.0;
+    int s = ch->outer_sst, ns = ch->sizes[s];
+    double lg = 0.0, t0 = 0.0;
+    memset(k, 0, sizeof(double) * (size_t)ch->nmax);
+    memset(dk, 0, sizeof(double) * (size_t)ch->nmax * ch->ntheta);
+    k[i0] = 1.0;
+    int j = 0;
+    for (; j < ch->nop && ch->op[j].tau <= bh; ++j) {
+        const oop_t *o = &ch->op[j];
+        int t = conn(ch, s, o->orb, o->spin, o->ca), nt = ch->sizes[t];
+        const double *E = ch->evals + ch->offsets[s];
+        prop_tangent(ch, s, o->tau - t0, k, dk, ns);
+        lg += -(E[0] - ch->egs) * (o->tau - t0);
+        apply_tangent(ch, opblk(ch, s, o->orb, o->spin, o->ca), nt, ns, k, dk);
+        split_log_tangent(ch, k, dk, nt, &lg);
+        s = t, ns = nt, t0 = o->tau;
+    }
+    {
+        const double *E = ch->evals + ch->offsets[s];
+        prop_tangent(ch, s, bh - t0, k, dk, ns);
+        lg += -(E[0] - ch->egs) * (bh - t0);
+        split_log_tangent(ch, k, dk, ns, &lg);
+    }
+    *kl = lg;
+    const int smid = s;
+    s = ch->outer_sst, ns = ch->sizes[s], lg = 0.0, t0 = ch->beta;
+    memset(b, 0, sizeof(double) * (size_t)ch->nmax);
+    memset(db, 0, sizeof(double) * (size_t)ch->nmax * ch->ntheta);
+    b[i0] = 1.0;
+    for (int q = ch->nop - 1; q >= j; --q) {
+        const oop_t *o = &ch->op[q];
+        int t = conn(ch, s, o->orb, o->spin, 3 - o->ca), nt = ch->sizes[t];
+        const double *E = ch->evals + ch->offsets[s];
+        prop_tangent(ch, s, t0 - o->tau, b, db, ns);
+        lg += -(E[0] - ch->egs) * (t0 - o->tau);
+        apply_tangent(ch, opblk(ch, s, o->orb, o->spin, 3 - o->ca), nt, ns, b, db);
+        split_log_tangent(ch, b, db, nt, &lg);
+        s = t, ns = nt, t0 = o->tau;
+    }
+    {
+        const double *E = ch->evals + ch->offsets[s];
+        prop_tangent(ch, s, t0 - bh, b, db, ns);
+        lg += -(E[0] - ch->egs) * (t0 - bh);
+        split_log_tangent(ch, b, db, ns, &lg);
+    }
+    *bl = lg;
+    return smid;
+}
+
+/* score-function sums of one measurement: s = d ln|w| / d theta;
+ * Sum sign s, Sum s x (this measurement's G(tau) bins), Sum s x (this
+ * measurement's density-matrix sample) and Sum d(sample)/dtheta at fixed
+ * configuration (the density matrix at beta/2 depends on H explicitly). */
+static void meas_score(ochain_t *ch)
+{
+    const int nth = ch->ntheta, nf = NF(ch);
+    double sc[8], dlt[8];
+    det_tangent(ch, sc);
+    trace_tangent(ch, dlt);
+    for (int t = 0; t < nth; ++t) {
+        sc[t] += dlt[t];
+        ch->sc_sum[t] += ch->cfgsign * sc[t];
+    }
+    /* G(tau) bins of this measurement, as meas_gtau */
+    double *gm = calloc((size_t)nf * ch->ntau, sizeof(double));
+    int cpos, apos[14];
+    memset(apos, 0, sizeof(apos));
+    for (int ia = 0; ia < ch->nop; ++ia) {
+        const oop_t *a = &ch->op[ia];
+        if (a->ca != 2) continue;
+        int f = FL(*a);
+        apos[f]++;
+        cpos = 0;
+        for (int ic = 0; ic < ch->nop; ++ic) {
+            const oop_t *c = &ch->op[ic];
+            if (FL(*c) != f || c->ca != 1) continue;
+            cpos++;
+            double M = ch->minv[f][(cpos - 1) + (long)(apos[f] - 1) * ch->k[f]];
+            int idt;
+            if (a->tau - c->tau > 0.0) {
+                idt = (int)((a->tau - c->tau) / ch->beta * (double)ch->ntau) + 1;
+            } else {
+                idt = (int)((ch->beta + a->tau - c->tau) / ch->beta * (double)ch->ntau) + 1;
+                M = -M;
+            }
+            if (idt > ch->ntau) idt = ch->ntau;
+            gm[(long)f * ch->ntau + idt - 1] += M * ch->cfgsign;
+        }
+    }
+    for (int t = 0; t < nth; ++t)
+        for (long i = 0; i < (long)nf * ch->ntau; ++i) ch->sc_g[(long)t * nf * ch->ntau + i] += sc[t] * gm[i];
+    free(gm);
+    /* density matrix at beta/2 and its explicit derivative */
+    if (ch->meas_dm) {
+        int no = ch->sizes[ch->outer_sst], nmax = ch->nmax;
+        long ns2 = (long)ch->nstates * ch->nstates, nst = ch->nstates;
+        double *k = malloc(sizeof(double) * nmax), *b = malloc(sizeof(double) * nmax);
+        double *dk = malloc(sizeof(double) * nmax * nth), *db = malloc(sizeof(double) * nmax * nth);
+        for (int i0 = 0; i0 < no; ++i0) {
+            double kl, bl;
+            int sm = half_tangent(ch, i0, k, dk, &kl, b, db, &bl);
+            int n = ch->sizes[sm], off = ch->offsets[sm];
+            double fac = ch->trace.sign * (-ch->cfgsign) * exp(kl + bl - ch->trace.log);
+            for (int s1 = 0; s1 < n; ++s1)
+                for (int s2 = 0; s2 < n; ++s2) {
+                    long e = (off + s1) + nst * (off + s2);
+                    double smp = fac * b[s1] * k[s2];
+                    for (int t = 0; t < nth; ++t) {
+                        ch->sc_dm[t * ns2 + e] += sc[t] * smp;
+                        ch->sc_dmx[t * ns2 + e] += fac * (db[(long)t * nmax + s1] * k[s2] + b[s1] * dk[(long)t * nmax + s2])
+                                                   - smp * dlt[t];
+                    }
+                }
+        }
+        free(k), free(b), free(dk), free(db);
+    }
+}
+
+/* Set the derivative tables: dblk [ntheta][sum n_s^2] (eigenbasis dH blocks,
+ * column-major per block in superstate order), dftau [ntheta][nf][nftau];
+ * score != 0 adds the score sums to every measurement. */
+int oracle_set_derivatives(void *h, int ntheta, const double *dblk, const double *dftau, int score)
+{
+    ochain_t *ch = h;
+    if (ntheta < 1 || ntheta > 8) return 1;
+    free(ch->dboff), free(ch->dblk), free(ch->dftau);
+    free(ch->sc_sum), free(ch->sc_g), free(ch->sc_dm), free(ch->sc_dmx);
+    ch->ntheta = ntheta;
+    ch->dboff = malloc(sizeof(long) * (size_t)ch->nsst);
+    long o = 0;
+    for (int s = 0; s < ch->nsst; ++s) ch->dboff[s] = o, o += (long)ch->sizes[s] * ch->sizes[s];
+    ch->dbsize = o;
+    ch->dblk = malloc(sizeof(double) * (size_t)(o * ntheta));
+    memcpy(ch->dblk, dblk, sizeof(double) * (size_t)(o * ntheta));
+    long nft = (long)NF(ch) * ch->nftau * ntheta;
+    ch->dftau = malloc(sizeof(double) * (size_t)nft);
+    memcpy(ch->dftau, dftau, sizeof(double) * (size_t)nft);
+    ch->dket = realloc(ch->dket, sizeof(double) * (size_t)ch->nmax * ntheta);
+    ch->score = score;
+    ch->sc_sum = calloc((size_t)ntheta, sizeof(double));
+    ch->sc_g = calloc((size_t)ntheta * NF(ch) * ch->ntau, sizeof(double));
+    ch->sc_dm = calloc((size_t)ntheta * ch->nstates * ch->nstates, sizeof(double));
+    ch->sc_dmx = calloc((size_t)ntheta * ch->nstates * ch->nstates, sizeof(double));
+    return 0;
+}
+
+/* d ln|det| / d theta and d ln|Tr| / d theta of the current configuration:
+ * out[2 t] determinant part, out[2 t + 1] trace part */
+void oracle_dlogw(void *h, double *out)
+{
+    ochain_t *ch = h;
+    double a[8], b[8];
+    det_tangent(ch, a);
+    trace_tangent(ch, b);
+    for (int t = 0; t < ch->ntheta; ++t) out[2 * t] = a[t], out[2 * t + 1] = b[t];
+}
+
+/* score sums: ssum [ntheta], g [ntheta][nf][ntau], dm and dmx [ntheta][nstates^2] */
+void oracle_score_sums(void *h, double *ssum, double *g, double *dm, double *dmx)
+{
+    ochain_t *ch = h;
+    long nt = ch->ntheta, ns2 = (long)ch->nstates * ch->nstates;
+    memcpy(ssum, ch->sc_sum, sizeof(double) * (size_t)nt);
+    memcpy(g, ch->sc_g, sizeof(double) * (size_t)(nt * NF(ch) * ch->ntau));
+    memcpy(dm, ch->sc_dm, sizeof(double) * (size_t)(nt * ns2));
+    memcpy(dmx, ch->sc_dmx, sizeof(double) * (size_t)(nt * ns2));
+}
+
 /* ---------------------------- public API --------------------------------- */
 typedef struct {
     int32_t kind, outcome, noper, outer_sst, window;
@@ -1406,6 +1734,8 @@ void oracle_destroy(void *h)
     free(ch->accpair), free(ch->aptau), free(ch->mstep), free(ch->dstep), free(ch->sstep);
     free(ch->sign_hist_sst), free(ch->tc_sst), free(ch->outer_state_hist);
     free(ch->sign_hist_states), free(ch->tc_states);
+    free(ch->dboff), free(ch->dblk), free(ch->dftau), free(ch->dket);
+    free(ch->sc_sum), free(ch->sc_g), free(ch->sc_dm), free(ch->sc_dmx);
     free(ch);
 }
 
@@ -1534,6 +1864,7 @@ void oracle_measure(void *h, long nmeas, long ncorr)
         meas_misc(ch);
         if (ch->meas_dm) meas_dm(ch);
         meas_gtau(ch);
+        if (ch->score) meas_score(ch);
     }
 }
 

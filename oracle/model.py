@@ -90,7 +90,38 @@ class OracleModel:
         self.connect = connect
 
         # H_loc blocks
+        self._psi = psi
+        h = self._hloc(umatrix, muimp, strict=False)
+        self.hblocks = [0.5 * (x + x.T) for x in h]
+
+        self.evals = []
+        self.evecs = []
+        for x in self.hblocks:
+            w, v = np.linalg.eigh(x)
+            self.evals.append(w)
+            self.evecs.append(v)
+        self.egs = min(w[0] for w in self.evals)
+
+        # eigenbasis operator blocks
+        self.psi_eb = {}
+        for (s1, b, sp, ca), m in psi.items():
+            s2 = connect[s1, b, sp, ca]
+            self.psi_eb[(s1, b, sp, ca)] = self.evecs[s2].T @ m @ self.evecs[s1]
+        self.psi_occ = psi
+
+    # ------------------------------------------------------------------
+    def _hloc(self, umatrix: np.ndarray, muimp: np.ndarray, strict: bool) -> list:
+        """Blocks of 1/2 U_1234 c+_1 c+_2 c_4 c_3 + muimp_12 c+_1 c_2 in the
+        occupation basis (``Operator.F90:481-542, 1041-1109``).  With
+        ``strict`` a term that leaves a block raises (a derivative that breaks
+        the quantum numbers)."""
+        n, nsst, connect, psi = self.nbands, self.nsst, self.connect, self._psi
         h = [np.zeros((k, k)) for k in self.sizes]
+
+        def leaves(s1, s):
+            if strict and s1 >= 0 and s1 != s:
+                raise ValueError("term maps superstate %d onto %d" % (s, s1))
+            return s1 != s
         for b1 in range(n):
             for sp1 in range(2):
                 for b2 in range(n):
@@ -103,7 +134,7 @@ class OracleModel:
                             if s2 < 0:
                                 continue
                             s1 = connect[s2, b1, sp1, 0]
-                            if s1 != s:
+                            if leaves(s1, s):
                                 continue
                             h[s] += v * psi[(s2, b1, sp1, 0)] @ psi[(s, b2, sp2, 1)]
         um = umatrix.reshape((n, 2) * 4)
@@ -121,26 +152,22 @@ class OracleModel:
                 if s2 < 0:
                     continue
                 s1 = connect[s2, b1, sp1, 0]
-                if s1 != s:
+                if leaves(s1, s):
                     continue
                 h[s] += v * (psi[(s2, b1, sp1, 0)] @ psi[(s3, b2, sp2, 0)]
                              @ psi[(s4, b4, sp4, 1)] @ psi[(s, b3, sp3, 1)])
-        self.hblocks = [0.5 * (x + x.T) for x in h]
+        return h
 
-        self.evals = []
-        self.evecs = []
-        for x in self.hblocks:
-            w, v = np.linalg.eigh(x)
-            self.evals.append(w)
-            self.evecs.append(v)
-        self.egs = min(w[0] for w in self.evals)
-
-        # eigenbasis operator blocks
-        self.psi_eb = {}
-        for (s1, b, sp, ca), m in psi.items():
-            s2 = connect[s1, b, sp, ca]
-            self.psi_eb[(s1, b, sp, ca)] = self.evecs[s2].T @ m @ self.evecs[s1]
-        self.psi_occ = psi
+    def derivative_blocks(self, du: np.ndarray, dmuimp: np.ndarray) -> np.ndarray:
+        """dH/dtheta in the eigenbasis, V_s^T (dH)_s V_s per superstate, for
+        inputs linear in (U matrix, muimp): the blocks concatenated column-major
+        at the density-matrix offsets (sum of n_s^2 entries)."""
+        h = self._hloc(np.asarray(du), np.asarray(dmuimp), strict=True)
+        out = []
+        for s in range(self.nsst):
+            x = 0.5 * (h[s] + h[s].T)
+            out.append(np.asfortranarray(self.evecs[s].T @ x @ self.evecs[s]).ravel(order="F"))
+        return np.concatenate(out)
 
     # ------------------------------------------------------------------
     def flat(self) -> dict:

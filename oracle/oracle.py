@@ -65,6 +65,10 @@ def lib():
         _lib.oracle_outer_histos.argtypes = [P, P, P, P, P, P]
         _lib.oracle_steps.argtypes = [P, P, P, P]
         _lib.oracle_steps.restype = C.c_int
+        _lib.oracle_set_derivatives.argtypes = [P, C.c_int, P, P, C.c_int]
+        _lib.oracle_set_derivatives.restype = C.c_int
+        _lib.oracle_dlogw.argtypes = [P, P]
+        _lib.oracle_score_sums.argtypes = [P, P, P, P, P]
     return _lib
 
 
@@ -224,6 +228,46 @@ class OracleChain:
         lib().oracle_set_config(self.h, len(taus), _p(taus), _p(orbs), _p(spins), _p(cas),
                                 int(outer_sst), _p(w))
         return dict(trace_log=w[0], trace_sign=w[1], det_log=w[2], det_sign=w[3], cfgsign=w[4])
+
+    # -- parameter derivatives (no reference counterpart) ----------------------
+    def set_derivatives(self, deriv, score: bool = True):
+        """Derivative tables of the inputs (an InputDerivatives: du, dmuimp,
+        dftau per theta); with ``score`` every measurement adds the
+        score-function sums."""
+        nt = len(deriv.du)
+        blk = np.concatenate([self.model.derivative_blocks(deriv.du[t], deriv.dmuimp[t])
+                              for t in range(nt)])
+        nb = self.nb
+        dft = np.zeros((nt, 2 * nb, deriv.dftau.shape[-1]))
+        for t in range(nt):
+            for b in range(nb):
+                for sp in range(2):
+                    dft[t, b + nb * sp] = deriv.dftau[t, b, sp, b, sp]
+        self._keep["dblk"] = np.ascontiguousarray(blk)
+        self._keep["dftau_d"] = np.ascontiguousarray(dft)
+        self.ntheta = nt
+        st = lib().oracle_set_derivatives(self.h, nt, _p(self._keep["dblk"]), _p(self._keep["dftau_d"]),
+                                          int(bool(score)))
+        assert st == 0
+
+    def dlogw(self) -> np.ndarray:
+        """[theta][0: d ln|det|, 1: d ln|Tr|] of the current configuration."""
+        out = np.zeros(2 * self.ntheta)
+        lib().oracle_dlogw(self.h, _p(out))
+        return out.reshape(self.ntheta, 2)
+
+    def score_sums(self) -> dict:
+        """Sum sign s, Sum s G-sample [theta][f][tau], Sum s rho-sample and
+        Sum d(rho-sample)/dtheta [theta][state][state] (eigenbasis, dense)."""
+        nt, nf, nst = self.ntheta, 2 * self.nb, 4 ** self.nb
+        ss = np.zeros(nt)
+        g = np.zeros(nt * nf * self.ntau)
+        dm = np.zeros(nt * nst * nst)
+        dmx = np.zeros(nt * nst * nst)
+        lib().oracle_score_sums(self.h, _p(ss), _p(g), _p(dm), _p(dmx))
+        return dict(ssum=ss, gtau=g.reshape(nt, nf, self.ntau),
+                    dm=dm.reshape(nt, nst, nst).transpose(0, 2, 1).copy(),
+                    dmx=dmx.reshape(nt, nst, nst).transpose(0, 2, 1).copy())
 
     def gtau_blocks(self) -> np.ndarray:
         """AccumGtau = 2: variances of the level-l bin means, [nlev][flavour][ntau]."""

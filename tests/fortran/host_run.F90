@@ -19,6 +19,7 @@ program host_run
    integer(c_int64_t) :: nwarm, nmoves
    real(c_double) :: taudiffmax, tdm
    real(c_double), allocatable :: umat(:,:,:,:), ftau(:,:,:,:,:), muimp(:,:,:,:), screen(:,:,:,:,:)
+   real(c_double), allocatable :: apt0(:)
    character(len=:), allocatable :: pstr
    character(len=600) :: line
 
@@ -63,6 +64,7 @@ program host_run
       call init_counters()
       call ctqmc_calibrate(.true., 1000, 100, 1000, 100000)
       if (aborted /= 0) stop 5
+      apt0 = AccPairTau          ! phase 2 deallocates AccPairTau, as the reference does
       tdm = AccPairTau(int(0.995d0*1000) + 1)
       call set_taudiffmax(tdm)
       call init_counters()
@@ -70,8 +72,8 @@ program host_run
       if (aborted /= 0) stop 6
       open (71, file=trim(dir)//'/calib.bin', access='stream', form='unformatted', status='replace', &
             action='write')
-      write (71) int(size(AccPairTau), c_int32_t)
-      write (71) AccPairTau
+      write (71) int(size(apt0), c_int32_t)
+      write (71) apt0
       write (71) tdm
       write (71) int(size(AccPair, 3), c_int32_t)
       write (71) AccPair

@@ -202,6 +202,8 @@ int compute_layout(const w2d_ctqmc *h, int nbound, int kc, w2d::ChainLayout &L)
     L.bndsst = take((long)nbound * 4);
     L.part = take(2L * nm * 8 + 16);
     L.kwork = take(nm > 16 ? 2L * nm * nm * 8 : 8);
+    const int nth = h->dm.ntheta;
+    L.dwork = take(nth > 0 ? ((4L + 3L * nth) * nm * nm + nm) * 8 : 8);
     L.ltau = take((long)L.lcap * 8);
     L.lcode = take((long)L.lcap * 4);
     L.gtau = take((long)nf * h->ntau * 8);
@@ -210,7 +212,7 @@ int compute_layout(const w2d_ctqmc *h, int nbound, int kc, w2d::ChainLayout &L)
     L.histo = take((long)nf * (h->maxhisto + 1) * 8);
     L.outer = take((long)h->model.nsst * 8);
     const bool bins = h->dm.accum == 2;
-    L.gbuf = take(bins ? (long)nf * h->ntau * 8 : 8);
+    L.gbuf = take((bins || h->dm.score) ? (long)nf * h->ntau * 8 : 8);
     L.gacc = take(bins ? 3L * h->dm.nlev * nf * h->ntau * 8 : 8);
     const long nst = h->model.nstates, nss = h->model.nsst;
     L.ohst = take(nst * 8);
@@ -226,6 +228,11 @@ int compute_layout(const w2d_ctqmc *h, int nbound, int kc, w2d::ChainLayout &L)
     L.sstep = take(h->dm.step_sign ? nms * 8 : 8);
     L.apt_cap = APT_CAP;
     L.aptau = take((long)L.apt_cap * 8);
+    const bool sc = h->dm.score && nth > 0;
+    L.scsum = take(sc ? nth * 8L : 8);
+    L.scg = take(sc ? (long)nth * nf * h->ntau * 8 : 8);
+    L.scdm = take(sc ? nth * h->dm.dmsize * 8 : 8);
+    L.scdmx = take(sc ? nth * h->dm.dmsize * 8 : 8);
     L.stride = o;
     // dynamic LDS
     const w2d::LocalModel &m = h->model;
@@ -265,6 +272,7 @@ int compute_layout(const w2d_ctqmc *h, int nbound, int kc, w2d::ChainLayout &L)
     L.l_row = lds((long)(kc + 2) * 8);
     L.l_pl = lds((long)(2 * nm + 2) * 8);
     L.l_k = nm <= 16 ? lds(2L * nm * nm * 8) : -1;
+    L.l_strip = nm > 16 ? lds(128L * 16 * 8) : -1;
     L.l_legc = lds(2L * std::max(h->nleg, 1) * 8);
     // blocks of up to 4 states: quad-walk chunk buffers (16 operators)
     if (nm <= 4) {
@@ -376,9 +384,17 @@ template <int NM>
 void launch_sweep(w2d_ctqmc *h, long long target, long long ncorr, int only, w2d_move_record *rec,
                   long long rec_base)
 {
-    allow_lds(w2d::sweep_kernel<NM>, h);
-    hipLaunchKernelGGL(w2d::sweep_kernel<NM>, dim3(h->nchains), dim3(w2d::WAVE), (unsigned)h->L.lds_bytes, h->stream, h->dm, h->L,
-                       h->chains, target, ncorr, only, rec, rec_base, (const int *)h->abort_dev, h->calib);
+    if (h->dm.score && h->dm.ntheta > 0) {
+        allow_lds(w2d::sweep_kernel<NM, true>, h);
+        hipLaunchKernelGGL((w2d::sweep_kernel<NM, true>), dim3(h->nchains), dim3(w2d::WAVE), (unsigned)h->L.lds_bytes,
+                           h->stream, h->dm, h->L, h->chains, target, ncorr, only, rec, rec_base,
+                           (const int *)h->abort_dev, h->calib);
+    } else {
+        allow_lds(w2d::sweep_kernel<NM, false>, h);
+        hipLaunchKernelGGL((w2d::sweep_kernel<NM, false>), dim3(h->nchains), dim3(w2d::WAVE), (unsigned)h->L.lds_bytes,
+                           h->stream, h->dm, h->L, h->chains, target, ncorr, only, rec, rec_base,
+                           (const int *)h->abort_dev, h->calib);
+    }
 }
 
 template <int NM>
@@ -387,6 +403,14 @@ void launch_setconfig(w2d_ctqmc *h, int chain)
     allow_lds(w2d::setconfig_kernel<NM>, h);
     hipLaunchKernelGGL(w2d::setconfig_kernel<NM>, dim3(h->nchains), dim3(w2d::WAVE), (unsigned)h->L.lds_bytes, h->stream, h->dm,
                        h->L, h->chains, chain);
+}
+
+template <int NM>
+void launch_dlogw(w2d_ctqmc *h, int chain, double *out)
+{
+    allow_lds(w2d::dlogw_kernel<NM>, h);
+    hipLaunchKernelGGL(w2d::dlogw_kernel<NM>, dim3(1), dim3(w2d::WAVE), (unsigned)h->L.lds_bytes, h->stream, h->dm,
+                       h->L, h->chains, chain, out);
 }
 
 #define DISPATCH(fn, ...)                    \
@@ -1089,6 +1113,87 @@ int w2d_ctqmc_destroy(w2d_ctqmc *h)
 {
     delete h;
     return W2D_OK;
+}
+
+int w2d_ctqmc_set_derivatives(w2d_ctqmc *h, int ntheta, const double *du, const double *dmuimp,
+                              const double *dftau, int score)
+{
+    if (!h || ntheta < 1 || ntheta > w2d::MAXTHETA || !du || !dmuimp || !dftau) return W2D_ERR_ARG;
+    const int nb = h->model.nbands, n2 = 2 * nb, nf = 2 * nb, nftau = h->dm.nftau;
+    const long usz = (long)n2 * n2 * n2 * n2, msz = (long)n2 * n2;
+    std::vector<double> blk, all;
+    for (int t = 0; t < ntheta; ++t) {
+        const std::string err = w2d::derivative_blocks(h->model, du + t * usz, dmuimp + t * msz, blk);
+        if (!err.empty()) {
+            std::fprintf(stderr, "w2d_ctqmc_set_derivatives: parameter %d: %s\n", t, err.c_str());
+            return W2D_ERR_ARG;
+        }
+        all.insert(all.end(), blk.begin(), blk.end());
+    }
+    // dF/dtheta(b, s, b, s, :) of each flavour f = b + N s, as the F(tau) tables
+    std::vector<double> fd((size_t)ntheta * nf * nftau);
+    for (int t = 0; t < ntheta; ++t)
+        for (int b = 0; b < nb; ++b)
+            for (int sp = 0; sp < 2; ++sp)
+                for (int k = 0; k < nftau; ++k) {
+                    const long idx = b + (long)nb * (sp + 2L * (b + (long)nb * (sp + 2L * k)));
+                    fd[((size_t)t * nf + b + nb * sp) * nftau + k] = dftau[(long)t * msz * nftau + idx];
+                }
+    int st;
+    const double *dblk = nullptr, *dft = nullptr;
+    if ((st = upload(h, all, &dblk)) || (st = upload(h, fd, &dft))) return st;
+    h->dm.dblk = dblk;
+    h->dm.dftau = dft;
+    h->dm.ntheta = ntheta;
+    h->dm.score = score != 0;
+    // room for the derivative work area and the score sums (cleared)
+    if ((st = relayout(h, h->kcap, h->L.nbound))) return st;
+    const w2d::ChainLayout &L = h->L;
+    HIPCHK(hipMemset2DAsync(h->chains + L.scsum, (size_t)L.stride, 0, (size_t)(L.stride - L.scsum),
+                            (size_t)h->nchains, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return W2D_OK;
+}
+
+int w2d_ctqmc_dlogw(w2d_ctqmc *h, int chain, double *out)
+{
+    if (!h || !out || chain < 0 || chain >= h->nchains) return W2D_ERR_ARG;
+    if (h->dm.ntheta < 1) return W2D_ERR_STATE;
+    double *d = nullptr;
+    HIPCHK(hipMalloc(&d, 2 * w2d::MAXTHETA * sizeof(double)));
+    DISPATCH(launch_dlogw, h, chain, d);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e == hipSuccess) e = hipMemcpy(out, d, 2 * h->dm.ntheta * sizeof(double), hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    HIPCHK(e);
+    return W2D_OK;
+}
+
+int w2d_ctqmc_get_score_sums(w2d_ctqmc *h, int ngroups, double *ssum, double *gtau, double *dm_blocks,
+                             double *dm_explicit)
+{
+    if (!h || ngroups < 1 || ngroups > h->nchains) return W2D_ERR_ARG;
+    if (!h->measured || !h->dm.score || h->dm.ntheta < 1) return W2D_ERR_STATE;
+    void *work = nullptr;
+    HIPCHK(hipMalloc(&work, group_sums_bytes(h, ngroups)));
+    int st = group_sums_into(h, ngroups, work, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+    const w2d::ChainLayout &L = h->L;
+    const long nel = (L.stride - L.gtau) / 8;
+    const double *part = static_cast<const double *>(work);
+    auto put = [&](double *dst, long off, long n) -> int {
+        if (!dst || n <= 0) return W2D_OK;
+        HIPCHK(hipMemcpy2D(dst, (size_t)n * 8, part + (off - L.gtau) / 8, (size_t)nel * 8, (size_t)n * 8,
+                           (size_t)ngroups, hipMemcpyDeviceToHost));
+        return W2D_OK;
+    };
+    const long nt = h->dm.ntheta, nf = 2 * h->model.nbands;
+    if (!st) st = put(ssum, L.scsum, nt);
+    if (!st) st = put(gtau, L.scg, nt * nf * h->ntau);
+    if (!st) st = put(dm_blocks, L.scdm, nt * h->dm.dmsize);
+    if (!st) st = put(dm_explicit, L.scdmx, nt * h->dm.dmsize);
+    (void)hipFree(work);
+    return st;
 }
 
 int w2d_ctqmc_get_group_sums(w2d_ctqmc *h, int ngroups, double *gtau, double *gleg, double *dm_blocks,

@@ -266,6 +266,7 @@ struct Ctx {
     double *ttau;        // trial list / staging buffer (LDS), same capacity
     int *tcode;
     double *K0, *K1;     // trace work matrices (LDS for small blocks)
+    double *strip;       // LDS strip of the state for the MFMA walks (blocks above 16 states)
     double *dvec, *Q, *R, *colb, *rowb, *pl;
     // window prefix kets rk[j] (after the first j window operators) and suffix
     // bras lb[m] (after the last m), with their logs and superstates (LDS)
@@ -859,6 +860,168 @@ DEV void apply_mfma(Ctx &c, const double *P, const double *Kin, double *Kout, in
     }
 }
 
+// What an operator walk over blocks above 16 states needs, by value (the walk
+// is out of line; a reference to the chain context would live in scratch).
+struct MfmaEnv {
+    const int *sz, *off;   // block sizes and first eigenvalue index per superstate
+    const int4 *tr;        // transition records (target, its size, block offset, its energies offset)
+    const double *ev, *ops;
+    double *dvec;          // LDS: propagator factors of the current block
+    double *strip;         // LDS: one 16-column strip of the state (128 x 16)
+    double egs;
+    int nb;
+};
+
+struct MfmaOut {
+    double lg;             // log scale added by the walk
+    double ntns, nss, nts; // algorithmic work: sum of nt*ns, ns, nt over the operators
+    int s;                 // superstate after the walk (-1: quantum-number violation)
+};
+
+DEV int4 mfma_trans(const MfmaEnv &e, int s, int c, bool adjoint)
+{
+    int ca = op_ca(c);
+    if (adjoint) ca = 3 - ca;
+    return e.tr[((s * e.nb + op_orb(c)) * 2 + op_spin(c)) * 2 + (ca - 1)];
+}
+
+// exp(-(E_k - E_0) dt) of superstate s into LDS (all lanes read it after the barrier)
+DEV void mfma_prop(const MfmaEnv &e, int s, double dt)
+{
+    const int ns = e.sz[s];
+    const double *E = e.ev + e.off[s];
+    const double e0 = E[0];
+    for (int k = lane_id(); k < ns; k += WAVE) e.dvec[k] = ns > 1 ? exp(-(E[k] - e0) * dt) : 1.0;
+    wsync();
+}
+
+// Walk of a state (ns x no, row-major, entry (r, i) at K[r * no + i]) over
+// operators otau/ocode (ascending tau; BRA: taken in descending order with the
+// adjoint blocks) from tstart to tend, for blocks above 16 states, on the FP64
+// matrix cores (v_mfma_f64_16x16x4f64).  The state is processed in strips of
+// 16 columns (outer states); a strip stays in LDS (rows x 16) through every
+// operator of the walk, so the state never goes back to memory between
+// operators.  Per operator and k-step of 4 rows, lane l reads the B entry
+// (row k0 + l/16, column l%16) of the strip times the source block's
+// propagator exp(-(E_k - E_0) dt), and one entry of the operator block (nt x
+// ns column-major) for each 16-row output tile; the output tiles accumulate
+// in registers and are written back to the strip.  Each strip is
+// renormalised by powers of two (exactly) when its largest entry leaves
+// [2^-64, 2^64]; at the end the strips are brought to a common power of two
+// (exactly) and written to Kout.  The common factors exp(-(E_0 - Egs) dt) go
+// into the returned log scale.
+template <bool BRA>
+__device__ __attribute__((noinline)) MfmaOut walk_mfma(const MfmaEnv e, const double *Kin, double *Kout, int s0,
+                                                      int no, double tstart, const double *otau, const int *ocode,
+                                                      int n, double tend)
+{
+    MfmaOut o{0.0, 0.0, 0.0, 0.0, -1};
+    const int l = lane_id(), lr = l & 15, lk = l >> 4;
+    double *sb = e.strip;   // LDS [128 rows][16 columns]
+    // superstate path, quantum numbers and the common log factor (uniform)
+    int s = s0;
+    double t = tstart;
+    for (int jj = 0; jj < n; ++jj) {
+        const int j = BRA ? n - 1 - jj : jj;
+        const double tj = otau[j];
+        const int4 tr = mfma_trans(e, s, ocode[j], BRA);
+        if (tr.x < 0) return o;
+        o.lg += -(e.ev[e.off[s]] - e.egs) * (BRA ? t - tj : tj - t);
+        const double ns = (double)e.sz[s], nt = (double)tr.y;
+        o.ntns += nt * ns;
+        o.nss += ns;
+        o.nts += nt;
+        s = tr.x;
+        t = tj;
+    }
+    o.lg += -(e.ev[e.off[s]] - e.egs) * (BRA ? t - tend : tend - t);
+    const int sfin = s, nsfin = e.sz[sfin], ns0 = e.sz[s0];
+    int emax = -2147483647, es_of[8];
+    for (int q = 0; q < 8; ++q) es_of[q] = 0;
+    for (int q = 0; q * 16 < no; ++q) {
+        for (int idx = l; idx < ns0 * 16; idx += WAVE) {
+            const int r = idx >> 4, i = q * 16 + (idx & 15);
+            sb[idx] = i < no ? Kin[(long)r * no + i] : 0.0;
+        }
+        wsync();
+        int es = 0, ns = ns0;
+        s = s0;
+        t = tstart;
+        for (int jj = 0; jj < n; ++jj) {
+            const int j = BRA ? n - 1 - jj : jj;
+            const double tj = otau[j];
+            const int4 tr = mfma_trans(e, s, ocode[j], BRA);
+            const int nt = tr.y, nkt = (ns + 15) >> 4, ntp = (nt + 15) >> 4;
+            mfma_prop(e, s, BRA ? t - tj : tj - t);
+            const double *P = e.ops + tr.z;
+            dbl4 out[8];
+#pragma unroll
+            for (int tp = 0; tp < 8; ++tp) out[tp] = dbl4{0.0, 0.0, 0.0, 0.0};
+            for (int kt = 0; kt < nkt; ++kt) {
+#pragma unroll
+                for (int ee = 0; ee < 4; ++ee) {
+                    const int k = 16 * kt + 4 * ee + lk;
+                    const bool kin = k < ns;
+                    const double b = kin ? sb[k * 16 + lr] * e.dvec[k] : 0.0;
+                    const double *Pk = P + (long)k * nt;
+#pragma unroll
+                    for (int tp = 0; tp < 8; ++tp) {
+                        if (tp < ntp) {
+                            const int r = 16 * tp + lr;
+                            const double a = (r < nt && kin) ? Pk[r] : 0.0;
+                            out[tp] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, out[tp], 0, 0, 0);
+                        }
+                    }
+                }
+            }
+            double m = 0.0;
+#pragma unroll
+            for (int tp = 0; tp < 8; ++tp)
+#pragma unroll
+                for (int ee = 0; ee < 4; ++ee)
+                    if (16 * tp + lk + 4 * ee < nt) m = fmax(m, fabs(out[tp][ee]));
+            int ex = 0;
+            if (__ballot(m >= STATE_TINY) == 0ull || __ballot(m > STATE_HUGE) != 0ull) {
+                const double mm = wave_max(m);
+                if (mm > 0.0) ex = ilogb(mm);
+            }
+            es += ex;
+            wsync();   // every lane has read the strip and dvec
+#pragma unroll
+            for (int tp = 0; tp < 8; ++tp)
+#pragma unroll
+                for (int ee = 0; ee < 4; ++ee) {
+                    const int r = 16 * tp + lk + 4 * ee;
+                    if (r < nt) sb[r * 16 + lr] = ex ? ldexp(out[tp][ee], -ex) : out[tp][ee];
+                }
+            wsync();
+            s = tr.x;
+            ns = nt;
+            t = tj;
+        }
+        // propagation from the last operator to tend, then the strip to Kout
+        mfma_prop(e, s, BRA ? t - tend : tend - t);
+        for (int idx = l; idx < ns * 16; idx += WAVE) {
+            const int r = idx >> 4, i = q * 16 + (idx & 15);
+            if (i < no) Kout[(long)r * no + i] = sb[idx] * e.dvec[r];
+        }
+        wsync();
+        es_of[q] = es;
+        emax = max(emax, es);
+    }
+    // one power of two for the whole state (exact)
+    for (int q = 0; q * 16 < no; ++q)
+        if (es_of[q] != emax)
+            for (int idx = l; idx < nsfin * 16; idx += WAVE) {
+                const int r = idx >> 4, i = q * 16 + (idx & 15);
+                if (i < no) Kout[(long)r * no + i] = ldexp(Kout[(long)r * no + i], es_of[q] - emax);
+            }
+    wsync();
+    o.lg += (double)emax * 0.69314718055994530942;
+    o.s = sfin;
+    return o;
+}
+
 // scale rows by dvec and renormalise (end-of-sweep propagation)
 DEV void scale_rows(Ctx &c, double *K, int ns, int no, double &lg)
 {
@@ -876,6 +1039,21 @@ DEV void scale_rows(Ctx &c, double *K, int ns, int no, double &lg)
         for (int idx = lane_id(); idx < ns * no; idx += WAVE) K[idx] *= inv;
     }
     wsync();
+}
+
+DEV MfmaEnv mfma_env(const Ctx &c)
+{
+    MfmaEnv e;
+    e.sz = c.sz, e.off = c.off, e.tr = c.tr, e.ev = c.ev, e.ops = c.opsp, e.dvec = c.dvec, e.strip = c.strip;
+    e.egs = c.M.egs, e.nb = c.M.nb;
+    return e;
+}
+
+// algorithmic work of an MFMA walk (as Work::commit counts it)
+DEV void mfma_work(Ctx &c, const MfmaOut &r, int no)
+{
+    c.wflops += (double)no * (2.0 * r.ntns + r.nss);
+    c.wbytes += 8.0 * no * (r.nss + r.nts);
 }
 
 // Ket sweep over a list (ascending tau) from t0 to t1.  Returns false if an
@@ -934,23 +1112,15 @@ DEV bool ket_walk(Ctx &c, double *&K, double *&Kw, int &s, double &lg, int no, d
         }
         finish_small<NM>(K, ns, no, d, ls, lg);
     } else {
-        for (int j = 0; j < n; ++j) {
-            const double tj = otau[j];
-            const int cj = ocode[j];
-            const int ts = c.conn(s, cj, false);
-            if (ts < 0) return w.commit(c), false;
-            lg += prop_factors(c, s, tj - t);
-            const int ns = c.sz[s], nt = c.sz[ts];
-            w.op(nt, ns, no);
-            apply_mfma(c, c.opblock(s, cj, false), K, Kw, nt, ns, no, lg);
-            double *x = K;
-            K = Kw;
-            Kw = x;
-            s = ts;
-            t = tj;
-        }
-        lg += prop_factors(c, s, t1 - t);
-        scale_rows(c, K, c.sz[s], no, lg);
+        const MfmaOut r = walk_mfma<false>(mfma_env(c), K, Kw, s, no, t, otau, ocode, n, t1);
+        mfma_work(c, r, no);
+        if (r.s < 0) return false;
+        lg += r.lg;
+        s = r.s;
+        double *x = K;
+        K = Kw;
+        Kw = x;
+        return true;
     }
     w.commit(c);
     return true;
@@ -1006,22 +1176,15 @@ DEV void bra_walk(Ctx &c, double *&B, double *&Bw, int &s, double &lg, int no, d
         }
         finish_small<NM>(B, ns, no, d, ls, lg);
     } else {
-        for (int j = n - 1; j >= 0; --j) {
-            const double tj = otau[j];
-            const int cj = ocode[j];
-            const int ts = c.conn(s, cj, true);
-            lg += prop_factors(c, s, t - tj);
-            const int ns = c.sz[s], nt = c.sz[ts];
-            w.op(nt, ns, no);
-            apply_mfma(c, c.opblock(s, cj, true), B, Bw, nt, ns, no, lg);
-            double *x = B;
-            B = Bw;
-            Bw = x;
-            s = ts;
-            t = tj;
-        }
-        lg += prop_factors(c, s, t - t0);
-        scale_rows(c, B, c.sz[s], no, lg);
+        const MfmaOut r = walk_mfma<true>(mfma_env(c), B, Bw, s, no, t, otau, ocode, n, t0);
+        mfma_work(c, r, no);
+        if (r.s < 0) return;
+        lg += r.lg;
+        s = r.s;
+        double *x = B;
+        B = Bw;
+        Bw = x;
+        return;
     }
     w.commit(c);
 }
@@ -2693,6 +2856,338 @@ DEV void meas_dm(Ctx &c, double *dm)
     }
 }
 
+// ---------------------------------------------------------------------------
+// parameter derivatives (no counterpart in the reference)
+// ---------------------------------------------------------------------------
+// d ln|w| / d theta of a configuration, w = det(Delta) Tr[...]:
+//  * d ln|det Delta| = sum_f tr(Delta_f^{-1} dDelta_f): each (creator,
+//    annihilator) pair of a flavour times dF/dtheta(tau_a - tau_c), looked up
+//    like get_HybrF (Trace.F90:1014-1033) on the derivative table;
+//  * d ln|Tr| by forward tangents: each state matrix K carries dK_t beside it.
+//    Over dt in block s (levels E_k relative to the block's lowest)
+//      K <- P K,   dK_t <- P dK_t - (D_t o Phi) K,
+//    P = diag(exp(-dt E_k)), D_t = V^T (dH/dtheta_t) V in the eigenbasis and
+//    Phi_kl = (exp(-dt E_k) - exp(-dt E_l)) / (E_l - E_k) (dt exp(-dt E_k) for
+//    equal levels); an operator block maps K and every dK_t alike, and one
+//    power of two rescales them together.  The ket from 0 and the bra from
+//    beta meet at beta/2 (as the density-matrix measurement), where the trace,
+//    its tangents, the density-matrix sample and that sample's explicit
+//    derivative are contracted.
+constexpr int MAXTHETA = 4;
+
+struct TanEnv {
+    const DevModel *M;
+    const int *sz, *off;
+    const int4 *tr;
+    const double *ev, *ops, *tau;
+    const int *code;
+    double *w;            // derivative work area of the chain
+    int nm, nth;
+};
+
+DEV TanEnv tan_env(const Ctx &c)
+{
+    TanEnv e;
+    e.M = &c.M, e.sz = c.sz, e.off = c.off, e.tr = c.tr, e.ev = c.ev, e.ops = c.opsp, e.tau = c.tau;
+    e.code = c.code, e.w = reinterpret_cast<double *>(c.base + c.L.dwork), e.nm = c.nm, e.nth = c.M.ntheta;
+    return e;
+}
+
+// divided difference of exp(-dt E) at two levels relative to the block's lowest
+DEV double phi_div(double ek, double el, double dt)
+{
+    const double a = fmin(ek, el), y = fabs(ek - el) * dt;
+    const double g = y > 0.0 ? -expm1(-y) / y : 1.0;
+    return dt * exp(-a * dt) * g;
+}
+
+DEV int4 tan_trans(const TanEnv &e, int s, int code, bool adjoint)
+{
+    int ca = op_ca(code);
+    if (adjoint) ca = 3 - ca;
+    return e.tr[((s * e.M->nb + op_orb(code)) * 2 + op_spin(code)) * 2 + (ca - 1)];
+}
+
+// segment of length dt in block s: dKw_t = P dK_t - (D_t o Phi) K, then K <- P K
+DEV void tan_prop(const TanEnv &e, int s, double dt, double *K, const double *dK, double *dKw, int no)
+{
+    const DevModel &M = *e.M;
+    const int ns = e.sz[s];
+    const long nm2 = (long)e.nm * e.nm;
+    const double *E = e.ev + e.off[s];
+    double *Phi = e.w, *pv = e.w + (4L + 3L * e.nth) * nm2;
+    for (int idx = lane_id(); idx < ns * ns; idx += WAVE)
+        Phi[idx] = phi_div(E[idx % ns] - E[0], E[idx / ns] - E[0], dt);
+    for (int k = lane_id(); k < ns; k += WAVE) pv[k] = exp(-(E[k] - E[0]) * dt);
+    wsync();
+    for (int t = 0; t < e.nth; ++t) {
+        const double *D = M.dblk + (long)t * M.dmsize + M.dmoff[s];
+        const double *dk = dK + t * nm2;
+        double *dw = dKw + t * nm2;
+        for (int idx = lane_id(); idx < ns * no; idx += WAVE) {
+            const int k = idx / no, i = idx - k * no;
+            double acc = pv[k] * dk[idx];
+            for (int l = 0; l < ns; ++l) acc = fma(-D[k + l * ns] * Phi[k + l * ns], K[l * no + i], acc);
+            dw[idx] = acc;
+        }
+    }
+    wsync();
+    for (int idx = lane_id(); idx < ns * no; idx += WAVE) K[idx] *= pv[idx / no];
+    wsync();
+}
+
+// operator block P (nt x ns, column-major): Kw = P K, dKw_t = P dK_t
+DEV void tan_apply(const TanEnv &e, const double *P, int nt, int ns, int no, const double *K, double *Kw,
+                   const double *dK, double *dKw)
+{
+    const long nm2 = (long)e.nm * e.nm;
+    for (int t = -1; t < e.nth; ++t) {
+        const double *src = t < 0 ? K : dK + t * nm2;
+        double *dst = t < 0 ? Kw : dKw + t * nm2;
+        for (int idx = lane_id(); idx < nt * no; idx += WAVE) {
+            const int r = idx / no, i = idx - r * no;
+            double acc = 0.0;
+            for (int c = 0; c < ns; ++c) acc = fma(P[r + c * nt], src[c * no + i], acc);
+            dst[idx] = acc;
+        }
+    }
+    wsync();
+}
+
+// one power of two for K and its tangents when their largest entry leaves [2^-64, 2^64]
+DEV void tan_renorm(const TanEnv &e, double *K, double *dK, int n, double &lg)
+{
+    const long nm2 = (long)e.nm * e.nm;
+    double m = 0.0;
+    for (int idx = lane_id(); idx < n; idx += WAVE) {
+        m = fmax(m, fabs(K[idx]));
+        for (int t = 0; t < e.nth; ++t) m = fmax(m, fabs(dK[t * nm2 + idx]));
+    }
+    const double mm = wave_max(m);
+    if (mm > 0.0 && (mm < STATE_TINY || mm > STATE_HUGE)) {
+        const int ex = ilogb(mm);
+        for (int idx = lane_id(); idx < n; idx += WAVE) {
+            K[idx] = ldexp(K[idx], -ex);
+            for (int t = 0; t < e.nth; ++t) dK[t * nm2 + idx] = ldexp(dK[t * nm2 + idx], -ex);
+        }
+        lg += (double)ex * 0.69314718055994530942;
+    }
+    wsync();
+}
+
+// Walk over operators [j0, j1) of the configuration (BRA: descending, adjoint
+// blocks) from tstart to tend with tangents; false on a quantum-number violation
+template <bool BRA>
+DEV bool tan_walk(const TanEnv &e, double *&K, double *&Kw, double *&dK, double *&dKw, int &s, double &lg, int no,
+                  double tstart, int j0, int j1, double tend)
+{
+    const DevModel &M = *e.M;
+    double t = tstart;
+    for (int jj = 0; jj < j1 - j0; ++jj) {
+        const int j = BRA ? j1 - 1 - jj : j0 + jj;
+        const double tj = e.tau[j];
+        const int4 tr = tan_trans(e, s, e.code[j], BRA);
+        if (tr.x < 0) return false;
+        const double dt = BRA ? t - tj : tj - t;
+        tan_prop(e, s, dt, K, dK, dKw, no);
+        lg += -(e.ev[e.off[s]] - M.egs) * dt;
+        tan_apply(e, e.ops + tr.z, tr.y, e.sz[s], no, K, Kw, dKw, dK);
+        double *x = K;
+        K = Kw, Kw = x;
+        tan_renorm(e, K, dK, tr.y * no, lg);
+        s = tr.x;
+        t = tj;
+    }
+    const double dt = BRA ? t - tend : tend - t;
+    tan_prop(e, s, dt, K, dK, dKw, no);
+    lg += -(e.ev[e.off[s]] - M.egs) * dt;
+    double *x = dK;
+    dK = dKw, dKw = x;
+    tan_renorm(e, K, dK, e.sz[s] * no, lg);
+    return true;
+}
+
+struct HalfTangents {
+    const double *K, *dK, *B, *dB;   // ket and bra at beta/2 (rows of block smid x outer states) and tangents
+    double kl, bl;                    // their log scales
+    int smid, no;
+    bool ok;
+};
+
+// ket from 0 and bra from beta to beta/2 with tangents
+DEV HalfTangents half_tangents(const TanEnv &e, int nop, int outer)
+{
+    const DevModel &M = *e.M;
+    const long nm2 = (long)e.nm * e.nm;
+    double *A0 = e.w + nm2, *A1 = A0 + nm2, *A2 = A1 + nm2;
+    double *T0 = A2 + nm2, *T1 = T0 + e.nth * nm2, *T2 = T1 + e.nth * nm2;
+    HalfTangents r;
+    const int no = e.sz[outer];
+    r.no = no;
+    const double bh = M.beta / 2.0;
+    const int nk = upper_bound_tau(e.tau, nop, bh);
+    double *K = A0, *Kw = A1, *dK = T0, *dKw = T1;
+    for (int idx = lane_id(); idx < no * no; idx += WAVE) {
+        K[idx] = (idx / no == idx % no) ? 1.0 : 0.0;
+        for (int t = 0; t < e.nth; ++t) dK[t * nm2 + idx] = 0.0;
+    }
+    wsync();
+    int s = outer;
+    r.kl = 0.0;
+    r.ok = tan_walk<false>(e, K, Kw, dK, dKw, s, r.kl, no, 0.0, 0, nk, bh);
+    r.smid = s;
+    const int nmid = e.sz[s];
+    for (int idx = lane_id(); idx < nmid * no; idx += WAVE) {
+        A2[idx] = K[idx];
+        for (int t = 0; t < e.nth; ++t) T2[t * nm2 + idx] = dK[t * nm2 + idx];
+    }
+    wsync();
+    double *B = A0, *Bw = A1, *dB = T0, *dBw = T1;
+    for (int idx = lane_id(); idx < no * no; idx += WAVE) {
+        B[idx] = (idx / no == idx % no) ? 1.0 : 0.0;
+        for (int t = 0; t < e.nth; ++t) dB[t * nm2 + idx] = 0.0;
+    }
+    wsync();
+    int sb = outer;
+    r.bl = 0.0;
+    r.ok = r.ok && tan_walk<true>(e, B, Bw, dB, dBw, sb, r.bl, no, M.beta, nk, nop, bh) && sb == s;
+    r.K = A2, r.dK = T2, r.B = B, r.dB = dB;
+    return r;
+}
+
+// d ln|Tr| / d theta from the beta/2 contraction
+DEV void trace_tangent(const TanEnv &e, const HalfTangents &h, double *dlt)
+{
+    const long nm2 = (long)e.nm * e.nm;
+    const int n = e.sz[h.smid] * h.no;
+    double tr = 0.0;
+    for (int idx = lane_id(); idx < n; idx += WAVE) tr += h.B[idx] * h.K[idx];
+    tr = wave_sum(tr);
+    for (int t = 0; t < e.nth; ++t) {
+        double d = 0.0;
+        for (int idx = lane_id(); idx < n; idx += WAVE)
+            d += h.dB[t * nm2 + idx] * h.K[idx] + h.B[idx] * h.dK[t * nm2 + idx];
+        dlt[t] = wave_sum(d) / tr;
+    }
+}
+
+// What a derivative pass needs, by value (the pass is out of line so that it
+// does not add to the sweep's registers; a reference to the chain context
+// would live in scratch).
+struct ScoreArgs {
+    TanEnv e;
+    const double *fc, *fa, *minv;
+    const int *fcs, *fas;
+    int kf[MAXF];
+    int kcap, nop, outer;
+    double cfgsign, trace_sign, trace_log;
+    double *scsum, *scg, *scdm, *scdmx;   // score sums of the chain
+    double *gbuf, *gtau;                  // this measurement's G(tau) bins, the G(tau) sums
+};
+
+DEV ScoreArgs score_args(const Ctx &c, double *gbuf, double *gtau)
+{
+    ScoreArgs a;
+    a.e = tan_env(c);
+    a.fc = c.fc, a.fa = c.fa, a.minv = c.minv, a.fcs = c.fcs, a.fas = c.fas;
+    for (int f = 0; f < MAXF; ++f) a.kf[f] = c.h.kf[f];
+    a.kcap = c.L.kcap, a.nop = c.h.nop, a.outer = c.h.outer;
+    a.cfgsign = c.h.cfgsign, a.trace_sign = c.h.trace_sign, a.trace_log = c.h.trace_log;
+    a.scsum = reinterpret_cast<double *>(c.base + c.L.scsum);
+    a.scg = reinterpret_cast<double *>(c.base + c.L.scg);
+    a.scdm = reinterpret_cast<double *>(c.base + c.L.scdm);
+    a.scdmx = reinterpret_cast<double *>(c.base + c.L.scdmx);
+    a.gbuf = gbuf, a.gtau = gtau;
+    return a;
+}
+
+// d ln|det| / d theta = sum_f tr(Delta_f^{-1} dDelta_f)
+DEV void det_tangent(const ScoreArgs &a, double *dld)
+{
+    const DevModel &M = *a.e.M;
+    const int ld = a.kcap;
+    double acc[MAXTHETA] = {0.0, 0.0, 0.0, 0.0};
+    for (int f = 0; f < M.nf; ++f) {
+        const int k = a.kf[f];
+        const double *ctl = a.fc + (long)f * ld, *atl = a.fa + (long)f * ld;
+        const int *csl = a.fcs + (long)f * ld, *asl = a.fas + (long)f * ld;
+        const double *Mf = a.minv + (long)f * ld * ld;
+        for (int p = lane_id(); p < k * k; p += WAVE) {
+            const int ai = p / k, ci = p - ai * k;
+            const double m = Mf[csl[ci] + (long)asl[ai] * ld];
+            double tau = atl[ai] - ctl[ci], sig = 1.0;
+            if (tau < 0.0) {
+                tau += M.beta;
+                sig = -1.0;
+            }
+            const double x = tau / M.beta * (double)(M.nftau - 1);
+            const int i = (int)x;
+            for (int t = 0; t < M.ntheta; ++t) {
+                const double *F = M.dftau + ((long)t * M.nf + f) * M.nftau;
+                acc[t] += m * sig * (F[i] + (x - (double)i) * (F[i + 1] - F[i]));
+            }
+        }
+    }
+    for (int t = 0; t < M.ntheta; ++t) dld[t] = wave_sum(acc[t]);
+}
+
+// One derivative pass over the configuration.  out != null: d ln|det| /
+// d theta_t -> out[2 t], d ln|Tr| / d theta_t -> out[2 t + 1].  accumulate:
+// the score-function sums of a measurement (s_t = d ln|w| / d theta_t):
+// Sum sign s_t, Sum s_t x this measurement's G(tau) bins (gbuf, which is then
+// added to the G(tau) sums unless the binning accumulator does that), Sum
+// s_t x density-matrix sample and Sum d(sample)/dtheta_t at fixed
+// configuration (the beta/2 sample depends on H explicitly).
+__device__ __attribute__((noinline)) void score_pass(const ScoreArgs a, double *out, bool accumulate)
+{
+    const DevModel &M = *a.e.M;
+    const TanEnv &e = a.e;
+    const long nm2 = (long)e.nm * e.nm;
+    double dld[MAXTHETA], dlt[MAXTHETA], sc[MAXTHETA];
+    det_tangent(a, dld);
+    const HalfTangents h = half_tangents(e, a.nop, a.outer);
+    trace_tangent(e, h, dlt);
+    if (out && lane_id() == 0)
+        for (int t = 0; t < M.ntheta; ++t) {
+            out[2 * t] = dld[t];
+            out[2 * t + 1] = h.ok ? dlt[t] : __builtin_nan("");
+        }
+    if (!accumulate) return;
+    for (int t = 0; t < M.ntheta; ++t) sc[t] = dld[t] + dlt[t];
+    if (lane_id() == 0)
+        for (int t = 0; t < M.ntheta; ++t) a.scsum[t] += a.cfgsign * sc[t];
+    if (M.meas_dm) {
+        const int n = e.sz[h.smid], no = h.no;
+        const long off = M.dmoff[h.smid];
+        const double fac = a.trace_sign * (-a.cfgsign) * exp(h.kl + h.bl - a.trace_log);
+        for (int idx = lane_id(); idx < n * n; idx += WAVE) {
+            const int s1 = idx % n, s2 = idx / n;
+            double smp = 0.0;
+            for (int i = 0; i < no; ++i) smp = fma(h.B[s1 * no + i], h.K[s2 * no + i], smp);
+            smp *= fac;
+            for (int t = 0; t < M.ntheta; ++t) {
+                double x = 0.0;
+                for (int i = 0; i < no; ++i)
+                    x += h.dB[t * nm2 + s1 * no + i] * h.K[s2 * no + i] + h.B[s1 * no + i] * h.dK[t * nm2 + s2 * no + i];
+                a.scdm[t * M.dmsize + off + idx] += sc[t] * smp;
+                a.scdmx[t * M.dmsize + off + idx] += fac * x - smp * dlt[t];
+            }
+        }
+    }
+    // this measurement's G(tau) bins: written by atomics at L2, read past the vector L1
+    wsync();
+    const long nc = (long)M.nf * M.ntau;
+    for (long i = lane_id(); i < nc; i += WAVE) {
+        const double v = __hip_atomic_load(a.gbuf + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int t = 0; t < M.ntheta; ++t) a.scg[t * nc + i] += sc[t] * v;
+        if (M.accum != 2) {
+            a.gtau[i] += v;
+            a.gbuf[i] = 0.0;
+        }
+    }
+    wsync();
+}
+
 // MeasSign, MeasOuterHistograms, MeasExpOrder (CTQMC.F90:3264-3350): visits
 // of the outer superstate and of each of its states, superstate sign, state
 // signs (trace sign x part-trace sign x configuration sign), trace
@@ -2797,6 +3292,7 @@ DEV Ctx make_ctx(const DevModel &M, const ChainLayout &L, char *base, int chain,
         c.K0 = reinterpret_cast<double *>(base + L.kwork);
     }
     c.K1 = c.K0 + (long)NM * NM;
+    c.strip = L.l_strip >= 0 ? reinterpret_cast<double *>(lds + L.l_strip) : nullptr;
     // header, RNG state and model tables into LDS
     c.ghdr = reinterpret_cast<ChainHdr *>(base - (long)chain * L.stride + L.hdrs) + chain;
     const int *gh = reinterpret_cast<const int *>(c.ghdr);
@@ -2870,7 +3366,9 @@ DEV void save_ctx(Ctx &c)
 // its signal flag before every move, CTQMC.F90:8212, 8436).  rec != nullptr
 // records every step of chain `only` at rec[pos - rec_base] (parity mode;
 // other chains return at once).
-template <int NM>
+// SCORE: the measurements add the score-function sums (a separate
+// instantiation, so that the derivative pass costs the plain sweep nothing)
+template <int NM, bool SCORE>
 __global__ void __launch_bounds__(WAVE) sweep_kernel(DevModel M, ChainLayout L, char *chains, long long target,
                                                      long long ncorr, int only, w2d_move_record *rec,
                                                      long long rec_base, const int *abort_word, int3 calib)
@@ -2936,7 +3434,8 @@ __global__ void __launch_bounds__(WAVE) sweep_kernel(DevModel M, ChainLayout L, 
             ga.nf = M.nf, ga.ntau = M.ntau, ga.nleg = M.nleg, ga.kcap = L.kcap;
             ga.beta = M.beta, ga.cfgsign = c.h.cfgsign, ga.legc = c.legc;
             ga.kf = c.h.kf, ga.fc = c.fc, ga.fa = c.fa, ga.minv = c.minv, ga.fcs = c.fcs, ga.fas = c.fas;
-            ga.gtau = M.accum == 2 ? gbuf : gtau, ga.gleg = gleg;
+            constexpr bool score = SCORE;
+            ga.gtau = (M.accum == 2 || score) ? gbuf : gtau, ga.gleg = gleg;
             const long long mi = c.h.cnt_meas - 1;   // g_inmeas - 1
             const bool st = mi < M.nmstep;
             ga.mstep = (st && M.step_mean) ? reinterpret_cast<double *>(base + L.gmstep) + mi * M.nf : nullptr;
@@ -2947,11 +3446,24 @@ __global__ void __launch_bounds__(WAVE) sweep_kernel(DevModel M, ChainLayout L, 
             const double2 wk = NM == 1 ? meas_gtau_narrow(ga) : meas_gtau(ga);
             c.wflops += wk.x;
             c.wbytes += wk.y;
+            if constexpr (SCORE) score_pass(score_args(c, gbuf, gtau), nullptr, true);
             if (M.accum == 2) autocorr_add(M, gbuf, gacc, gtau, c.h.cnt_meas - 1);
             if (__hip_atomic_load(abort_word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)) c.h.halt = HALT_ABORT;
         }
     }
     save_ctx(c);
+}
+
+// d ln|det| / d theta and d ln|Tr| / d theta of one chain's configuration:
+// out[2 t], out[2 t + 1]
+template <int NM>
+__global__ void __launch_bounds__(WAVE) dlogw_kernel(DevModel M, ChainLayout L, char *chains, int chain,
+                                                       double *out)
+{
+    extern __shared__ __align__(16) char lds[];
+    char *base = chains + (long)chain * L.stride;
+    Ctx c = make_ctx<NM>(M, L, base, chain, lds);
+    score_pass(score_args(c, nullptr, nullptr), out, false);
 }
 
 // Copy every chain from layout A to layout B (larger capacity): header, random

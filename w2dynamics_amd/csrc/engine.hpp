@@ -28,6 +28,11 @@ struct DevModel {
     int accum, nlev;                    // AccumGtau (1 mean, 2 binning) and its levels
     int nmstep;                         // length of the step-resolved arrays (Nmeas), 0 = off
     int step_mean, step_mid, step_sign; // Gtau_mean_step, Gtau_mid_step, sign_step
+    // parameter derivatives: dH/dtheta blocks in the eigenbasis [ntheta][dmsize]
+    // (at dmoff, column-major) and dF/dtheta [ntheta][nf][nftau]; score != 0
+    // adds the score-function sums to every measurement
+    int ntheta, score;
+    const double *dblk, *dftau;
 };
 
 // per-chain scalar state (kept contiguously for all chains, after the chain blocks)
@@ -80,6 +85,10 @@ struct ChainLayout {
     long accpair;         // accepted pairs per flavour and distance bin [nf][naccpair_cap]
     long gmstep, gdstep, sstep;   // step-resolved G(tau) mean / mid, sign [nmstep][nf], [nmstep]
     long aptau;           // accepted-pair distances [apt_cap] (calibration)
+    // score-function sums [ntheta]: sign x s, s x G-sample [nf][ntau], s x
+    // rho-sample and d(rho-sample)/dtheta [dmsize]; s = d ln|w| / d theta
+    long scsum, scg, scdm, scdmx;
+    long dwork;           // derivative work: (4 + 3 ntheta) matrices nmax x nmax and nmax factors
     int naccpair_cap, apt_cap;
     int cap, kcap, nbound, lcap;
     long kmat;            // max(kcap*kcap, nmax*nmax)
@@ -90,6 +99,7 @@ struct ChainLayout {
     // prefix kets / suffix bras of the window's operators (blocks up to 4 states)
     int sc;               // window operators they cover (0: not kept)
     long l_rk, l_lb, l_rkx, l_lbx, l_rks, l_lbs, l_qa, l_qlc, l_qrec, l_qsst;
+    long l_strip;         // 16-column strip of the state for the MFMA walks (blocks above 16 states)
 };
 
 }  // namespace w2d

@@ -141,6 +141,58 @@ static int apply_op(int state, int bit, bool dag, double &sign)
     return state ^ (1 << bit);
 }
 
+// Blocks of muimp_12 c+_1 c_2 + 1/2 U_1234 c+_1 c+_2 c_4 c_3 in the occupation
+// basis (Operator.F90:481-542, 1041-1109).  Terms that map a block onto
+// another are dropped, or with `strict` reported (returns false).
+static bool hamiltonian_blocks(int nb, const double *umat, const double *muimp, const std::vector<int> &s2s,
+                               const std::vector<std::vector<int>> &blk, const std::vector<int> &index_in_block,
+                               const std::vector<int> &sizes, bool strict, std::vector<std::vector<double>> &h)
+{
+    const int nsst = (int)blk.size();
+    bool ok = true;
+    h.assign(nsst, {});
+    for (int s = 0; s < nsst; ++s) h[s].assign((size_t)sizes[s] * sizes[s], 0.0);
+    auto add_term = [&](double val, const int *bits, const bool *dag, int nops) {
+        for (int s = 0; s < nsst; ++s)
+            for (int st : blk[s]) {
+                int cur = st;
+                double sg = 1.0;
+                for (int o = nops - 1; o >= 0 && cur >= 0; --o) cur = apply_op(cur, bits[o], dag[o], sg);
+                if (cur < 0) continue;
+                if (s2s[cur] != s) {
+                    if (strict) ok = false;
+                    continue;
+                }
+                int n = sizes[s];
+                h[s][index_in_block[cur] + (size_t)index_in_block[st] * n] += val * sg;
+            }
+    };
+    const int n2 = 2 * nb;
+    for (int b1 = 0; b1 < nb; ++b1)
+        for (int s1 = 0; s1 < 2; ++s1)
+            for (int b2 = 0; b2 < nb; ++b2)
+                for (int s2 = 0; s2 < 2; ++s2) {
+                    double v = muimp[b1 + nb * s1 + 2 * nb * b2 + 2 * nb * nb * s2];
+                    if (v == 0.0) continue;
+                    int bits[2] = {b1 + nb * s1, b2 + nb * s2};
+                    bool dag[2] = {true, false};
+                    add_term(v, bits, dag, 2);
+                }
+    for (int i = 0; i < n2; ++i)
+        for (int j = 0; j < n2; ++j)
+            for (int k = 0; k < n2; ++k)
+                for (int l = 0; l < n2; ++l) {
+                    double u = umat[i + (size_t)n2 * (j + (size_t)n2 * (k + (size_t)n2 * l))];
+                    if (u == 0.0) continue;
+                    // flavour index 2*b + s (CTQMC.F90 u_matrix layout)
+                    int b[4] = {i / 2, j / 2, k / 2, l / 2}, sp[4] = {i % 2, j % 2, k % 2, l % 2};
+                    int bits[4] = {b[0] + nb * sp[0], b[1] + nb * sp[1], b[3] + nb * sp[3], b[2] + nb * sp[2]};
+                    bool dag[4] = {true, true, false, false};
+                    add_term(0.5 * u, bits, dag, 4);
+                }
+    return ok;
+}
+
 std::string build_local_model(int nb, const double *umat, const double *muimp,
                               const std::string &qns, LocalModel &m)
 {
@@ -219,42 +271,8 @@ std::string build_local_model(int nb, const double *umat, const double *muimp,
                     m.connect[cidx(s, b, sp, ca0)] = target;
                 }
     // H_loc per block: one-particle term plus 1/2 U_1234 c+_1 c+_2 c_4 c_3
-    std::vector<std::vector<double>> h(nsst);
-    for (int s = 0; s < nsst; ++s) h[s].assign((size_t)m.sizes[s] * m.sizes[s], 0.0);
-    auto add_term = [&](double val, const int *bits, const bool *dag, int nops) {
-        for (int s = 0; s < nsst; ++s)
-            for (int st : blk[s]) {
-                int cur = st;
-                double sg = 1.0;
-                for (int o = nops - 1; o >= 0 && cur >= 0; --o) cur = apply_op(cur, bits[o], dag[o], sg);
-                if (cur < 0 || s2s[cur] != s) continue;
-                int n = m.sizes[s];
-                h[s][index_in_block[cur] + (size_t)index_in_block[st] * n] += val * sg;
-            }
-    };
-    const int n2 = 2 * nb;
-    for (int b1 = 0; b1 < nb; ++b1)
-        for (int s1 = 0; s1 < 2; ++s1)
-            for (int b2 = 0; b2 < nb; ++b2)
-                for (int s2 = 0; s2 < 2; ++s2) {
-                    double v = muimp[b1 + nb * s1 + 2 * nb * b2 + 2 * nb * nb * s2];
-                    if (v == 0.0) continue;
-                    int bits[2] = {b1 + nb * s1, b2 + nb * s2};
-                    bool dag[2] = {true, false};
-                    add_term(v, bits, dag, 2);
-                }
-    for (int i = 0; i < n2; ++i)
-        for (int j = 0; j < n2; ++j)
-            for (int k = 0; k < n2; ++k)
-                for (int l = 0; l < n2; ++l) {
-                    double u = umat[i + (size_t)n2 * (j + (size_t)n2 * (k + (size_t)n2 * l))];
-                    if (u == 0.0) continue;
-                    // flavour index 2*b + s (CTQMC.F90 u_matrix layout)
-                    int b[4] = {i / 2, j / 2, k / 2, l / 2}, sp[4] = {i % 2, j % 2, k % 2, l % 2};
-                    int bits[4] = {b[0] + nb * sp[0], b[1] + nb * sp[1], b[3] + nb * sp[3], b[2] + nb * sp[2]};
-                    bool dag[4] = {true, true, false, false};
-                    add_term(0.5 * u, bits, dag, 4);
-                }
+    std::vector<std::vector<double>> h;
+    hamiltonian_blocks(nb, umat, muimp, s2s, blk, index_in_block, m.sizes, false, h);
     // force_hermitian, diagonalise, eigenbasis
     m.evals.assign(ns, 0.0);
     m.evoff.assign(nsst, 0);
@@ -308,6 +326,47 @@ std::string build_local_model(int nb, const double *umat, const double *muimp,
                     m.opoff[cidx(s, b, sp, ca0)] = (long)m.ops.size();
                     m.ops.insert(m.ops.end(), out.begin(), out.end());
                 }
+    return "";
+}
+
+std::string derivative_blocks(const LocalModel &m, const double *dumat, const double *dmuimp,
+                              std::vector<double> &out)
+{
+    const int nsst = m.nsst;
+    std::vector<std::vector<int>> blk(nsst);
+    std::vector<int> index_in_block(m.nstates);
+    for (int s = 0; s < nsst; ++s)
+        for (int i = 0; i < m.sizes[s]; ++i) {
+            const int st = m.states[m.offsets[s] + i];
+            blk[s].push_back(st);
+            index_in_block[st] = i;
+        }
+    std::vector<std::vector<double>> h;
+    if (!hamiltonian_blocks(m.nbands, dumat, dmuimp, m.state_sst, blk, index_in_block, m.sizes, true, h))
+        return "the derivative of H_loc does not conserve the quantum numbers";
+    out.clear();
+    for (int s = 0; s < nsst; ++s) {
+        const int n = m.sizes[s];
+        const double *V = m.evecs.data() + m.evoff[s];
+        std::vector<double> a = h[s], t((size_t)n * n, 0.0);
+        for (int i = 0; i < n; ++i)          // symmetric part, as force_hermitian
+            for (int j = i + 1; j < n; ++j) {
+                double x = 0.5 * (a[i + (size_t)j * n] + a[j + (size_t)i * n]);
+                a[i + (size_t)j * n] = a[j + (size_t)i * n] = x;
+            }
+        for (int j = 0; j < n; ++j)          // t = a V
+            for (int k = 0; k < n; ++k) {
+                const double v = V[k + (size_t)j * n];
+                if (v == 0.0) continue;
+                for (int i = 0; i < n; ++i) t[i + (size_t)j * n] += a[i + (size_t)k * n] * v;
+            }
+        for (int j = 0; j < n; ++j)          // V^T t
+            for (int i = 0; i < n; ++i) {
+                double acc = 0.0;
+                for (int k = 0; k < n; ++k) acc += V[k + (size_t)i * n] * t[k + (size_t)j * n];
+                out.push_back(acc);
+            }
+    }
     return "";
 }
 

@@ -54,4 +54,10 @@ struct LocalModel {
 std::string build_local_model(int nbands, const double *umatrix_2n4, const double *muimp_n2n2,
                               const std::string &quantum_numbers, LocalModel &m);
 
+// d H_loc / d theta in the eigenbasis for inputs linear in (U matrix, muimp):
+// V_s^T (H(dumat, dmuimp))_s V_s per block, column-major, concatenated in block
+// order (sum of n_s^2 entries).  Error if the derivative leaves the blocks.
+std::string derivative_blocks(const LocalModel &m, const double *dumat, const double *dmuimp,
+                              std::vector<double> &out);
+
 }  // namespace w2d

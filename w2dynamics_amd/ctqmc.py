@@ -92,6 +92,9 @@ def load_library(path: str | None = None):
     lib.w2d_ctqmc_get_group_histograms.argtypes = [P, I, P, P, P, P, P, P]
     lib.w2d_ctqmc_get_steps.argtypes = [P, I, C.POINTER(I), P, P, P]
     lib.w2d_ctqmc_get_group_sums_noalloc.argtypes = [P, I, P, P, P, P, P, P, P]
+    lib.w2d_ctqmc_set_derivatives.argtypes = [P, I, P, P, P, I]
+    lib.w2d_ctqmc_dlogw.argtypes = [P, I, P]
+    lib.w2d_ctqmc_get_score_sums.argtypes = [P, I, P, P, P, P]
     lib.w2d_ctqmc_version.restype = C.c_char_p
     _lib = lib
     return lib
@@ -387,6 +390,80 @@ class CtqmcGpu:
         sums = {k: v[sums["nused"] > 0] for k, v in sums.items()}
         check_accumulators(sums)
         return pooled_estimate(sums, self.nbands, self.beta, self.ntau, self.nleg, self.model())
+
+
+    # -- parameter derivatives (no counterpart in the reference) -----------------
+    def set_derivatives(self, deriv, score: bool = True):
+        """Derivatives of the create inputs with respect to parameters theta (an
+        ``problems.InputDerivatives``: du, dmuimp, dftau per theta, 1..4 of
+        them); with ``score`` every later measurement adds the score sums."""
+        nt = len(deriv.du)
+        n2 = 2 * self.nbands
+        du = np.ascontiguousarray(np.stack([np.asarray(d).reshape(n2, n2, n2, n2).ravel(order="F")
+                                            for d in deriv.du]))
+        dmu = np.ascontiguousarray(np.stack([np.asarray(d).ravel(order="F") for d in deriv.dmuimp]))
+        dft = np.ascontiguousarray(np.stack([np.asarray(d).ravel(order="F") for d in deriv.dftau]))
+        _check(self.lib.w2d_ctqmc_set_derivatives(self.h, nt, _p(du), _p(dmu), _p(dft), int(bool(score))),
+               "set_derivatives")
+        self.ntheta = nt
+        self._deriv = deriv
+
+    def dlogw(self, chain: int = 0) -> np.ndarray:
+        """[theta][0: d ln|det Delta|, 1: d ln|Tr|] of chain ``chain``'s configuration."""
+        out = np.zeros(2 * self.ntheta)
+        _check(self.lib.w2d_ctqmc_dlogw(self.h, int(chain), _p(out)), "dlogw")
+        return out.reshape(self.ntheta, 2)
+
+    def score_sums(self, ngroups: int) -> dict:
+        """Score sums of the last measure over ``ngroups`` groups of chains:
+        ssum [g][theta], gtau [g][theta][f][tau], dm and dmx [g][theta][state][state]
+        (eigenbasis, block-diagonal)."""
+        nt, nf, ng = self.ntheta, 2 * self.nbands, int(ngroups)
+        m = self.model()
+        dmsize = int((m["sizes"].astype(np.int64) ** 2).sum())
+        ss = np.zeros((ng, nt))
+        g = np.zeros((ng, nt, nf, self.ntau))
+        db = np.zeros((ng, nt, dmsize))
+        dx = np.zeros((ng, nt, dmsize))
+        _check(self.lib.w2d_ctqmc_get_score_sums(self.h, ng, _p(ss), _p(g), _p(db), _p(dx)), "get_score_sums")
+        out = dict(ssum=ss, gtau=g)
+        for key, blk in (("dm", db), ("dmx", dx)):
+            dm = np.zeros((ng, nt, self.nstates, self.nstates))
+            o = 0
+            for off, n in zip(m["offsets"], m["sizes"]):
+                dm[:, :, off:off + n, off:off + n] = blk[:, :, o:o + n * n].reshape(ng, nt, n, n).swapaxes(2, 3)
+                o += n * n
+            out[key] = dm
+        return out
+
+    def score_estimate(self, ngroups: int = 64) -> dict:
+        """Observables and their derivatives d<O>/dtheta (score-function
+        estimates) pooled over all completed chains, jackknife errors over
+        groups of chains; see ``derivatives.score_estimate``."""
+        from .derivatives import score_estimate
+        ok = int((self.chain_status()["status"] == 0).sum())
+        ng = max(1, min(int(ngroups), ok))
+        sums = self.group_sums(ng)
+        sc = self.score_sums(ng)
+        keep = sums["nused"] > 0
+        sums = {k: v[keep] for k, v in sums.items()}
+        sc = {k: v[keep] for k, v in sc.items()}
+        check_accumulators(sums)
+        return score_estimate(sums, sc, self.nbands, self.beta, self.ntau, self.model(),
+                              self._ftau_diag(), _dftau_diag(self._deriv, self.nbands))
+
+    def _ftau_diag(self) -> np.ndarray:
+        return _diag_flavours(self._inputs[1], self.nbands)
+
+
+def _diag_flavours(ftau, nb: int) -> np.ndarray:
+    """F(b, s, b, s, :) of each flavour f = b + N s."""
+    ftau = np.asarray(ftau)
+    return np.stack([ftau[b, s, b, s] for s in range(2) for b in range(nb)])
+
+
+def _dftau_diag(deriv, nb: int) -> np.ndarray:
+    return np.stack([_diag_flavours(d, nb) for d in deriv.dftau])
 
 
 def _param_value(params: str, key: str) -> float:

@@ -1,0 +1,105 @@
+"""Score-function derivatives of CT-HYB observables with respect to model
+parameters theta (no counterpart in the reference).
+
+With configuration weights w(C; theta) (the reference's det(Delta) x local
+trace, ``MatrixUpdate.F90:41-99``, ``Trace.F90:5734-6102``) and signed
+Monte Carlo averages <X> = Sum sign X / Sum sign,
+
+    d<O>/dtheta = <O s> - <O><s> + <dO/dtheta>,   s = d ln|w| / dtheta,
+
+where the last term is the explicit dependence of the estimator itself.  The
+engine accumulates, per chain, Sum sign s, Sum s x (G(tau) sample), Sum s x
+(density-matrix sample) and Sum d(density-matrix sample)/dtheta at fixed
+configuration (``CtqmcGpu.score_sums``).  The G(tau) estimator depends on theta
+only through Delta: where dF/dtheta = lambda_f F_f for every flavour (no bath
+dependence: lambda = 0; a bath coupling V with F ~ V^2: lambda = 2/V), its
+explicit term is -lambda_f <G_f>; for other dF/dtheta it is not formed and
+dG/dtheta is NaN.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+
+def _proportionality(dftau: np.ndarray, ftau: np.ndarray, rtol: float = 1e-12) -> np.ndarray:
+    """lambda[t, f] with dF_t,f = lambda F_f (0 if dF vanishes, NaN if not proportional)."""
+    nt, nf = dftau.shape[:2]
+    lam = np.full((nt, nf), np.nan)
+    for t in range(nt):
+        for f in range(nf):
+            d, F = dftau[t, f], ftau[f]
+            scale = np.abs(d).max()
+            if scale == 0.0:
+                lam[t, f] = 0.0
+                continue
+            x = float(d @ F / (F @ F))
+            if np.abs(d - x * F).max() <= rtol * max(scale, 1e-300):
+                lam[t, f] = x
+    return lam
+
+
+def _estimate(tot: dict, sc: dict, nb: int, beta: float, ntau: int, model: dict, lam: np.ndarray) -> dict:
+    S = tot["scalars"][0]
+    nt = sc["ssum"].shape[0]
+    ss = sc["ssum"] / S                                   # <s>
+    nf = 2 * nb
+
+    def fl2(a):   # [..., f, x] -> [..., b, s, x]
+        return a.reshape(a.shape[:-2] + (2, nb, a.shape[-1])).swapaxes(-3, -2)
+
+    g = tot["gtau"] / S * ntau / beta ** 2                # [f][tau]
+    dg = (sc["gtau"] / S * ntau / beta ** 2 - g[None] * ss[:, None, None]
+          - lam[:, :, None] * g[None])                    # [t][f][tau]
+    dm = -tot["dm"] / S
+    ddm = -(sc["dm"] / S - tot["dm"][None] / S * ss[:, None, None] + sc["dmx"] / S)
+    dm_occ = dm.copy()
+    ddm_occ = ddm.copy()
+    for k, (o, n) in enumerate(zip(model["offsets"], model["sizes"])):
+        v = model["evecs"][k]
+        dm_occ[o:o + n, o:o + n] = v @ dm[o:o + n, o:o + n] @ v.T
+        ddm_occ[:, o:o + n, o:o + n] = v @ ddm[:, o:o + n, o:o + n] @ v.T
+    states = np.asarray(model["states"])
+    nocc = np.array([[(states >> (b + nb * s)) & 1 for s in range(2)] for b in range(nb)])
+    occ = np.einsum("k,ijk,lmk->ijlm", np.diagonal(dm_occ), nocc, nocc)
+    docc_t = np.einsum("tk,ijk,lmk->tijlm", np.diagonal(ddm_occ, axis1=1, axis2=2), nocc, nocc)
+    b = np.arange(nb)
+    return dict(gtau=fl2(g), dgtau=fl2(dg), occ=occ, docc=docc_t,
+                double_occ=occ[b, 0, b, 1], ddouble_occ=docc_t[:, b, 0, b, 1], dlogw_mean=ss,
+                mean_sign=S / tot["scalars"][1])
+
+
+def score_estimate(sums: dict, score: dict, nb: int, beta: float, ntau: int, model: dict,
+                   ftau: np.ndarray, dftau: np.ndarray) -> dict:
+    """Pooled observables and score-function derivatives over the leading
+    (group) axis of ``sums`` (``CtqmcGpu.group_sums``) and ``score``
+    (``CtqmcGpu.score_sums``), with jackknife errors over that axis
+    (``<key>_err``).  ftau [f][tau] and dftau [theta][f][tau] are the
+    hybridisation tables per flavour f = b + N s.
+
+    Keys: gtau [b][s][tau], dgtau [theta][b][s][tau] (NaN where dF/dtheta is
+    not proportional to F), occ [b][s][b][s] and docc [theta][...],
+    double_occ [b] = <n_b,up n_b,down> and ddouble_occ [theta][b],
+    dlogw_mean [theta] = <d ln|w| / dtheta>."""
+    lam = _proportionality(np.asarray(dftau, float), np.asarray(ftau, float))
+    keys = ("gtau", "dm", "scalars")
+    skeys = ("ssum", "gtau", "dm", "dmx")
+    tot = {k: np.asarray(sums[k], float).sum(0) for k in keys}
+    stot = {k: np.asarray(score[k], float).sum(0) for k in skeys}
+    out = _estimate(tot, stot, nb, beta, ntau, model, lam)
+    n = len(sums["scalars"])
+    if n < 2:
+        for k in list(out):
+            out[k + "_err"] = np.full_like(np.asarray(out[k], float), np.nan)
+        return out
+    jk = {k: [] for k in out}
+    for i in range(n):
+        t = {k: tot[k] - np.asarray(sums[k], float)[i] for k in keys}
+        st = {k: stot[k] - np.asarray(score[k], float)[i] for k in skeys}
+        r = _estimate(t, st, nb, beta, ntau, model, lam)
+        for k in out:
+            jk[k].append(np.asarray(r[k], float))
+    for k in list(out):
+        a = np.array(jk[k])
+        out[k + "_err"] = np.sqrt((n - 1) / n * ((a - a.mean(0)) ** 2).sum(0))
+    return out

@@ -178,10 +178,10 @@ module MCTQMC
       end function
       integer(c_int) function w2d_ctqmc_get_pair_stats(h, chain, apt, napt, ap, nap) &
             bind(C, name="w2d_ctqmc_get_pair_stats")
-         import :: c_int, c_ptr, c_double
+         import :: c_int, c_ptr
          type(c_ptr), value :: h
          integer(c_int), value :: chain
-         real(c_double) :: apt(*), ap(*)
+         type(c_ptr), value :: apt, ap     ! c_null_ptr: sizes only
          integer(c_int) :: napt, nap
       end function
       integer(c_int) function w2d_ctqmc_get_group_histograms(h, ng, ost, ssst, sst, csst, cst, ap) &
@@ -196,14 +196,14 @@ module MCTQMC
          type(c_ptr), value :: h
          integer(c_int), value :: chain
          integer(c_int) :: nm
-         real(c_double) :: ms(*), ds(*), ss(*)
+         type(c_ptr), value :: ms, ds, ss  ! c_null_ptr: size only
       end function
       integer(c_int) function w2d_ctqmc_get_gtau_blocks(h, chain, nl, blocks) bind(C, name="w2d_ctqmc_get_gtau_blocks")
          import :: c_int, c_ptr, c_double
          type(c_ptr), value :: h
          integer(c_int), value :: chain
          integer(c_int) :: nl
-         real(c_double) :: blocks(*)
+         type(c_ptr), value :: blocks      ! c_null_ptr: size only
       end function
       integer(c_int) function w2d_ctqmc_destroy(h) bind(C, name="w2d_ctqmc_destroy")
          import :: c_int, c_ptr
@@ -359,7 +359,7 @@ contains
       integer(8) :: t0, t1, rate
       integer :: c
       integer(c_int) :: napt, nap, lflag
-      real(KINDR), allocatable :: lst(:), ap(:,:)
+      real(KINDR), allocatable, target :: lst(:)
       call system_clock(t0, rate)
       lflag = merge(1, 0, list)
       if (failed(w2d_ctqmc_calibrate(handle, lflag, int(phase1pairnum, c_int), int(phase2taugrid, c_int), &
@@ -373,7 +373,14 @@ contains
          AccPairTau = 0
          do c = 0, n_chains - 1
             lst = 0
-            if (failed(w2d_ctqmc_get_pair_stats(handle, int(c, c_int), lst, napt, lst, nap), &
+            if (failed(w2d_ctqmc_get_pair_stats(handle, int(c, c_int), c_null_ptr, napt, c_null_ptr, nap), &
+                       "w2d_ctqmc_get_pair_stats")) return
+            if (napt > phase1pairnum) then
+               write (*, '(A)') 'MCTQMC: accepted-pair list longer than phase1pairnum'
+               aborted = 2
+               return
+            end if
+            if (failed(w2d_ctqmc_get_pair_stats(handle, int(c, c_int), c_loc(lst), napt, c_null_ptr, nap), &
                        "w2d_ctqmc_get_pair_stats")) return
             call sort_ascending(lst(1:napt))
             AccPairTau = AccPairTau + lst/n_chains
@@ -388,14 +395,14 @@ contains
    subroutine collect_accpair()
       integer :: c
       integer(c_int) :: napt, nap
-      real(KINDR) :: dum(1)
-      real(KINDR), allocatable :: ap(:,:)
-      if (failed(w2d_ctqmc_get_pair_stats(handle, 0_c_int, dum, napt, dum, nap), "w2d_ctqmc_get_pair_stats")) return
+      real(KINDR), allocatable, target :: ap(:,:)
+      if (failed(w2d_ctqmc_get_pair_stats(handle, 0_c_int, c_null_ptr, napt, c_null_ptr, nap), &
+                 "w2d_ctqmc_get_pair_stats")) return
       if (allocated(AccPair)) deallocate (AccPair)
       allocate (AccPair(nb, 2, nap), ap(nap, 2*nb))
       AccPair = 0
       do c = 0, n_chains - 1
-         if (failed(w2d_ctqmc_get_pair_stats(handle, int(c, c_int), dum, napt, ap, nap), &
+         if (failed(w2d_ctqmc_get_pair_stats(handle, int(c, c_int), c_null_ptr, napt, c_loc(ap), nap), &
                     "w2d_ctqmc_get_pair_stats")) return
          AccPair(:, 1, :) = AccPair(:, 1, :) + transpose(ap(:, 1:nb))/n_chains
          AccPair(:, 2, :) = AccPair(:, 2, :) + transpose(ap(:, nb + 1:2*nb))/n_chains
@@ -610,13 +617,13 @@ contains
    subroutine collect_steps_and_blocks()
       integer(c_int) :: nm, nl
       integer :: c, b, s, f
-      real(KINDR), allocatable :: ms(:,:), ds(:,:), ss(:), bl(:,:,:)
-      real(KINDR) :: dum(1)
+      real(KINDR), allocatable, target :: ms(:,:), ds(:,:), ss(:), bl(:,:,:)
       if (allocated(Gtau_mean_step)) deallocate (Gtau_mean_step)
       if (allocated(Gtau_mid_step)) deallocate (Gtau_mid_step)
       if (allocated(sign_step)) deallocate (sign_step)
       if (allocated(gtau_blocks)) deallocate (gtau_blocks)
-      if (failed(w2d_ctqmc_get_steps(handle, 0_c_int, nm, dum, dum, dum), "w2d_ctqmc_get_steps")) return
+      if (failed(w2d_ctqmc_get_steps(handle, 0_c_int, nm, c_null_ptr, c_null_ptr, c_null_ptr), &
+                 "w2d_ctqmc_get_steps")) return
       if (nm > 0) then
          allocate (ms(2*nb, nm), ds(2*nb, nm), ss(nm))
          allocate (Gtau_mean_step(nb, 2, nm), Gtau_mid_step(nb, 2, nm), sign_step(nm))
@@ -624,7 +631,8 @@ contains
          Gtau_mid_step = 0
          sign_step = 0
          do c = 0, n_chains - 1
-            if (failed(w2d_ctqmc_get_steps(handle, int(c, c_int), nm, ms, ds, ss), "w2d_ctqmc_get_steps")) return
+            if (failed(w2d_ctqmc_get_steps(handle, int(c, c_int), nm, c_loc(ms), c_loc(ds), c_loc(ss)), &
+                       "w2d_ctqmc_get_steps")) return
             do s = 1, 2
                do b = 1, nb
                   f = b + nb*(s - 1)
@@ -635,11 +643,12 @@ contains
             sign_step = sign_step + ss/n_chains
          end do
       end if
-      if (w2d_ctqmc_get_gtau_blocks(handle, 0_c_int, nl, dum) == 0) then
+      if (w2d_ctqmc_get_gtau_blocks(handle, 0_c_int, nl, c_null_ptr) == 0) then
          allocate (bl(ntau, 2*nb, nl), gtau_blocks(nb, 2, ntau, nl))
          gtau_blocks = 0
          do c = 0, n_chains - 1
-            if (failed(w2d_ctqmc_get_gtau_blocks(handle, int(c, c_int), nl, bl), "w2d_ctqmc_get_gtau_blocks")) return
+            if (failed(w2d_ctqmc_get_gtau_blocks(handle, int(c, c_int), nl, c_loc(bl)), &
+                       "w2d_ctqmc_get_gtau_blocks")) return
             do s = 1, 2
                do b = 1, nb
                   f = b + nb*(s - 1)

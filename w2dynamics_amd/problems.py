@@ -397,3 +397,83 @@ def coulomb_5orb(beta=40.0, f0=4.0, j=0.8, f4_over_f2=0.625, filling=6, mu=None,
     p.update(params)
     return Problem(5, beta, um, semicircle_ftau(5, beta, 2.0, nftau),
                    solver_muimp(5, mu, cf), "Nt Szt", "Coulomb", p, "5orb-coulomb")
+
+
+# --------------------------------------------------------------------------
+# parameter derivatives of the solver inputs
+# --------------------------------------------------------------------------
+
+
+@dataclass
+class InputDerivatives:
+    """Derivatives of the ``init_solver`` inputs with respect to model
+    parameters theta: ``du[t]`` of the U matrix (N,2,N,2,N,2,N,2), ``dmuimp[t]``
+    of the one-particle term (N,2,N,2), ``dftau[t]`` of F(tau) (N,2,N,2,Nftau)."""
+    names: tuple
+    du: np.ndarray
+    dmuimp: np.ndarray
+    dftau: np.ndarray
+
+    @property
+    def ntheta(self) -> int:
+        return len(self.names)
+
+
+#: the parameters of the differentiable 3-orbital Kanamori model
+KANAMORI_THETA = ("U", "J", "eps_d", "V")
+
+
+def kanamori_3orb_theta(theta=(3.5, 0.6, 0.0, 1.0), beta=100.0, mu=2.55, half_bandwidth=2.0,
+                        nftau=2001, seed=3, **params):
+    """The 3-orbital Kanamori problem as a function of theta = (U, J, eps_d, V),
+    and the derivatives of its solver inputs.
+
+    * U, J: Kanamori interaction with U' = U - 2J (``interaction.py:221-244``);
+      the U matrix is linear in (U, J).
+    * eps_d: a common shift of the three orbital levels on top of the seeded
+      crystal field; the one-particle term gains eps_d on the diagonal.
+    * V: bath coupling, F(tau) = V^2 int de rho_D(e) / (exp(-tau e) +
+      exp((beta - tau) e)), so dF/dV = 2 F / V (V = D/2 is the Bethe value).
+    * mu is held fixed (2.55 = 1.5 (U - 3J) at U = 3.5, J = 0.6).
+    """
+    u, j, eps, v = (float(x) for x in theta)
+    cf = np.random.default_rng(seed).uniform(-0.1, 0.1, 3)
+    p = {"Ntau": 1000, "NLegMax": 100}
+    p.update(params)
+    ftau1 = semicircle_ftau(3, beta, half_bandwidth, nftau, v2=1.0)
+    prob = Problem(3, beta, u_kanamori(3, u, u - 2 * j, j), v * v * ftau1,
+                   solver_muimp(3, mu, cf + eps), "Nt Szt", "Kanamori", p, "3orb-kanamori-theta")
+    nb = 3
+    du = np.stack([u_kanamori(nb, 1.0, 1.0, 0.0), u_kanamori(nb, 0.0, -2.0, 1.0),
+                   np.zeros((nb, 2) * 4), np.zeros((nb, 2) * 4)])
+    dmu = np.zeros((4, nb, 2, nb, 2))
+    for b in range(nb):
+        for s in range(2):
+            dmu[2, b, s, b, s] = 1.0
+    dft = np.zeros((4,) + ftau1.shape)
+    dft[3] = 2.0 * v * ftau1
+    return prob, InputDerivatives(KANAMORI_THETA, du, dmu, dft)
+
+
+#: the parameters of the differentiable 1-orbital Anderson model
+ANDERSON_THETA = ("U", "eps_d", "V")
+
+
+def anderson_discrete_theta(theta=(2.0, -0.6, 1.0), beta=5.0, eps_k=(-1.0, 0.2, 1.1),
+                            v_k=(0.5, 0.35, 0.45), nftau=4001, **params):
+    """1-orbital Anderson impurity with discrete bath sites as a function of
+    theta = (U, eps_d, V): H_loc = eps_d (n_up + n_dn) + U n_up n_dn, bath
+    couplings V v_k (F ~ V^2, dF/dV = 2 F / V); exactly solvable by
+    diagonalising impurity + bath."""
+    u, eps, v = (float(x) for x in theta)
+    p = {"Ntau": 1000, "NLegMax": 100}
+    p.update(params)
+    f1 = discrete_bath_ftau(beta, np.asarray(eps_k, float), np.asarray(v_k, float), nftau)
+    prob = Problem(1, beta, u_density(1, u, 0.0, 0.0), v * v * f1, solver_muimp(1, -eps), "Nt Szt",
+                   "Density", p, "1orb-anderson-discrete-theta")
+    du = np.stack([u_density(1, 1.0, 0.0, 0.0), np.zeros((1, 2) * 4), np.zeros((1, 2) * 4)])
+    dmu = np.zeros((3, 1, 2, 1, 2))
+    dmu[1, 0, 0, 0, 0] = dmu[1, 0, 1, 0, 1] = 1.0
+    dft = np.zeros((3,) + f1.shape)
+    dft[2] = 2.0 * v * f1
+    return prob, InputDerivatives(ANDERSON_THETA, du, dmu, dft)
