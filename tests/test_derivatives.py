@@ -190,3 +190,34 @@ def test_score_estimator_on_an_exactly_weighted_ensemble():
         nocc = np.array([[(states >> (b + nb * s_)) & 1 for s_ in range(2)] for b in range(nb)])
         docc = np.einsum("k,ijk,lmk->ijlm", np.diagonal(fdd), nocc, nocc)
         assert np.allclose(est["docc"][t], docc, rtol=1e-6, atol=1e-8)
+
+
+def test_autograd_function_applies_the_score_jacobian():
+    """CtqmcObservables: forward returns the run's estimates, backward the
+    vector-Jacobian product with dG/dtheta and d(double occupancy)/dtheta;
+    NaN Jacobian entries only matter where the incoming gradient is non-zero."""
+    import torch
+    from w2dynamics_amd.autograd import observables
+    rng = np.random.default_rng(5)
+    nb, ntau, nt = 2, 7, 3
+    est = dict(gtau=rng.normal(size=(nb, 2, ntau)), double_occ=rng.normal(size=nb),
+               dgtau=rng.normal(size=(nt, nb, 2, ntau)), ddouble_occ=rng.normal(size=(nt, nb)))
+    est["dgtau"][2, 1] = np.nan                  # not provided for theta_2, orbital 1
+
+    seen = []
+
+    def run(theta):
+        seen.append(np.array(theta))
+        return est
+
+    theta = torch.tensor([1.0, 2.0, 3.0], dtype=torch.float64, requires_grad=True)
+    g, d = observables(theta, run)
+    assert np.allclose(seen[0], [1.0, 2.0, 3.0])
+    assert np.allclose(g.detach().numpy(), est["gtau"]) and np.allclose(d.detach().numpy(), est["double_occ"])
+    w = rng.normal(size=(nb, 2, ntau))
+    w[1] = 0.0
+    loss = (g * torch.as_tensor(w)).sum() + 3.0 * d.sum()
+    loss.backward()
+    dg = np.nan_to_num(est["dgtau"])
+    want = np.einsum("tbsk,bsk->t", dg, w) + 3.0 * est["ddouble_occ"].sum(1)
+    assert np.allclose(theta.grad.numpy(), want, rtol=1e-12, atol=1e-12)

@@ -136,3 +136,34 @@ def test_score_estimates_match_exact_diagonalisation():
         dev = np.abs(g + dg) / (ge + 5e-4)
         print(name, "dG largest deviation in sigma", dev.max(), flush=True)
         assert np.mean(dev <= 3) >= 0.95 and dev.max() <= 4.5, (name, dev.max())
+
+
+def test_autograd_gradients_match_crn_finite_differences():
+    """3-orbital Kanamori, theta = (U, J, eps_d, V), 1024 chains: gradients that
+    the torch.autograd binding back-propagates (score-function Jacobian) for
+    the double occupancies (w.r.t. U) and G(tau) (w.r.t. J) against central
+    differences with common random numbers (same seeds, same warm start):
+    within 3 sigma at delta = 0.05 and within 2 sigma at delta = 1e-3, the
+    latter dominated by its own noise."""
+    import torch
+    from w2dynamics_amd.autograd import ScoreRun, observables
+    from w2dynamics_amd.derivatives import finite_difference_crn
+
+    def builder(th):
+        return P.kanamori_3orb_theta(th, beta=10.0, Ntau=20, NLegMax=10, NCorr=20, Nmeas=1000, nftau=1001)
+
+    nchains, seed = 1024, 777
+    run = ScoreRun(builder, nchains=nchains, seed=seed, nwarm=4000, nmeas=1000)
+    theta = torch.tensor(KAN0, dtype=torch.float64, requires_grad=True)
+    g, d = observables(theta, run)
+    est = run.last
+    # gradient of the summed double occupancy: the autograd path against the estimate
+    d.sum().backward()
+    assert np.allclose(theta.grad.numpy(), est["ddouble_occ"].sum(1), rtol=1e-12, atol=1e-12)
+    for t, key, delta, nsig in ((0, "double_occ", 0.05, 3.0), (1, "gtau", 0.05, 3.0), (0, "double_occ", 1e-3, 2.0)):
+        fd = finite_difference_crn(builder, KAN0, t, delta, run.configs, nchains, seed + 1, 1000)
+        sc, se = est["d" + key][t], est["d" + key + "_err"][t]
+        dev = np.abs(sc - fd[key]) / np.sqrt(se ** 2 + fd[key + "_err"] ** 2 + 1e-30)
+        print(P.KANAMORI_THETA[t], key, "delta", delta, "score", np.ravel(sc)[:3], "fd", np.ravel(fd[key])[:3],
+              "largest deviation in sigma", dev.max(), flush=True)
+        assert np.mean(dev <= nsig) >= 0.9 and dev.max() <= nsig + 1.5, (t, key, delta, dev.max())

@@ -298,7 +298,9 @@ int compute_layout(const w2d_ctqmc *h, int nbound, int kc, w2d::ChainLayout &L)
     // blocks of up to 4 states: the kernels read operator blocks from LDS only
     // operator blocks join LDS when the total stays within 64 KB (more would
     // cut the number of chains resident per CU); blocks of up to 4 states always
-    L.l_ops = (nm <= 4 || q + opsb <= 64 * 1024) ? lds(opsb) : -1;
+    // (above 16 states they stay in global memory: the MFMA walks read them
+    // with global instructions)
+    L.l_ops = (nm <= 4 || (nm <= 16 && q + opsb <= 64 * 1024)) ? lds(opsb) : -1;
     L.lds_bytes = q;
     if (L.lds_bytes > h->lds_max) return W2D_ERR_CAPACITY;
     L.hdrs = L.stride * h->nchains;
@@ -770,6 +772,24 @@ int w2d_ctqmc_create(const char *params, int nbands, int nftau, const double *u_
     std::vector<int> first_state(m.nsst);
     for (int s = 0; s < m.nsst; ++s) first_state[s] = m.states[m.offsets[s]];
     std::vector<int> opoff32(m.opoff.begin(), m.opoff.end());
+    // zero-padded copies of the operator blocks for the MFMA walks (blocks above 16 states)
+    std::vector<double> ops16;
+    std::vector<int> opoff16(m.opoff.size(), -1);
+    if (h->nm > 16) {
+        for (size_t i = 0; i < m.connect.size(); ++i) {
+            const int ts = m.connect[i];
+            if (ts < 0 || m.opoff[i] < 0) continue;
+            const int src = (int)(i / (4 * (size_t)nbands));   // superstate of this record
+            const int nt = m.sizes[ts], ns = m.sizes[src];
+            const int ntp = (nt + 15) / 16 * 16, nsp = (ns + 15) / 16 * 16;
+            opoff16[i] = (int)ops16.size();
+            ops16.resize(ops16.size() + (size_t)ntp * nsp, 0.0);
+            double *dst = ops16.data() + opoff16[i];
+            const double *srcb = m.ops.data() + m.opoff[i];
+            for (int k = 0; k < ns; ++k)
+                for (int r = 0; r < nt; ++r) dst[r + (size_t)k * ntp] = srcb[r + (size_t)k * nt];
+        }
+    }
     std::vector<int4> trans(m.connect.size());
     for (size_t i = 0; i < m.connect.size(); ++i) {
         const int ts = m.connect[i];
@@ -782,7 +802,8 @@ int w2d_ctqmc_create(const char *params, int nbands, int nftau, const double *u_
         (st = upload(h.get(), m.ops, &d.ops)) || (st = upload(h.get(), m.evals, &d.evals)) ||
         (st = upload(h.get(), fdiag, &d.ftau)) || (st = upload(h.get(), dmoff, &d.dmoff)) ||
         (st = upload(h.get(), legc, &d.legc)) || (st = upload(h.get(), trans, &d.trans)) ||
-        (st = upload(h.get(), opoff32, &d.opoff32)))
+        (st = upload(h.get(), opoff32, &d.opoff32)) || (st = upload(h.get(), ops16, &d.ops16)) ||
+        (st = upload(h.get(), opoff16, &d.opoff16)))
         return st;
     d.p_shift = P.real("PercentageTauShiftMove");
     d.p_glob = P.real("PercentageGlobalMove");

@@ -866,6 +866,8 @@ struct MfmaEnv {
     const int *sz, *off;   // block sizes and first eigenvalue index per superstate
     const int4 *tr;        // transition records (target, its size, block offset, its energies offset)
     const double *ev, *ops;
+    const double *ops16;   // zero-padded operator blocks (multiples of 16 x 16)
+    const int *off16;      // their offsets, indexed like tr
     double *dvec;          // LDS: propagator factors of the current block
     double *strip;         // LDS: one 16-column strip of the state (128 x 16)
     double egs;
@@ -878,21 +880,68 @@ struct MfmaOut {
     int s;                 // superstate after the walk (-1: quantum-number violation)
 };
 
-DEV int4 mfma_trans(const MfmaEnv &e, int s, int c, bool adjoint)
+DEV int mfma_tidx(const MfmaEnv &e, int s, int c, bool adjoint)
 {
     int ca = op_ca(c);
     if (adjoint) ca = 3 - ca;
-    return e.tr[((s * e.nb + op_orb(c)) * 2 + op_spin(c)) * 2 + (ca - 1)];
+    return ((s * e.nb + op_orb(c)) * 2 + op_spin(c)) * 2 + (ca - 1);
 }
 
-// exp(-(E_k - E_0) dt) of superstate s into LDS (all lanes read it after the barrier)
+DEV int4 mfma_trans(const MfmaEnv &e, int s, int c, bool adjoint) { return e.tr[mfma_tidx(e, s, c, adjoint)]; }
+
+// exp(-(E_k - E_0) dt) of superstate s into LDS, zero up to the next multiple
+// of 16 (all lanes read it after the barrier)
 DEV void mfma_prop(const MfmaEnv &e, int s, double dt)
 {
     const int ns = e.sz[s];
     const double *E = e.ev + e.off[s];
     const double e0 = E[0];
-    for (int k = lane_id(); k < ns; k += WAVE) e.dvec[k] = ns > 1 ? exp(-(E[k] - e0) * dt) : 1.0;
+    for (int k = lane_id(); k < (ns + 15) / 16 * 16; k += WAVE)
+        e.dvec[k] = k < ns ? (ns > 1 ? exp(-(E[k] - e0) * dt) : 1.0) : 0.0;
     wsync();
+}
+
+typedef __attribute__((address_space(3))) double lds_d;
+typedef __attribute__((address_space(1))) double glb_d;
+
+// out[tp] (tp < NTP) += P diag(dvec) S over nkt groups of 16 k-steps: P the
+// zero-padded operator block (leading dimension ldp, global), S the LDS strip
+// (rows beyond the block zero).  For v_mfma_f64_16x16x4f64 lane l holds
+// A(l%16, l/16) = P(16 tp + l%16, k), B(l/16, l%16) = S(k, l%16), k = k0 + l/16.
+// The A fragments of group kt + 1 are loaded before the MFMAs of group kt, so
+// that their latency overlaps matrix work.
+template <int NTP>
+DEV void mfma_kloop(dbl4 (&out)[8], const glb_d *P, int ldp, int nkt, const lds_d *sb, const lds_d *dvec, int lr,
+                    int lk)
+{
+    double a0[4][NTP], a1[4][NTP];
+    auto load_a = [&](int kt, double (&a)[4][NTP]) {
+#pragma unroll
+        for (int ee = 0; ee < 4; ++ee) {
+            const glb_d *Pk = P + (long)(16 * kt + 4 * ee + lk) * ldp + lr;
+#pragma unroll
+            for (int tp = 0; tp < NTP; ++tp) a[ee][tp] = Pk[16 * tp];
+        }
+    };
+    auto step = [&](int kt, const double (&a)[4][NTP]) {
+#pragma unroll
+        for (int ee = 0; ee < 4; ++ee) {
+            const int k = 16 * kt + 4 * ee + lk;
+            const double b = sb[k * 16 + lr] * dvec[k];
+#pragma unroll
+            for (int tp = 0; tp < NTP; ++tp)
+                out[tp] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ee][tp], b, out[tp], 0, 0, 0);
+        }
+    };
+    load_a(0, a0);
+    int kt = 0;
+    for (; kt + 1 < nkt; kt += 2) {
+        load_a(kt + 1, a1);
+        step(kt, a0);
+        if (kt + 2 < nkt) load_a(kt + 2, a0);
+        step(kt + 1, a1);
+    }
+    if (kt < nkt) step(kt, a0);
 }
 
 // Walk of a state (ns x no, row-major, entry (r, i) at K[r * no + i]) over
@@ -917,7 +966,14 @@ __device__ __attribute__((noinline)) MfmaOut walk_mfma(const MfmaEnv e, const do
 {
     MfmaOut o{0.0, 0.0, 0.0, 0.0, -1};
     const int l = lane_id(), lr = l & 15, lk = l >> 4;
-    double *sb = e.strip;   // LDS [128 rows][16 columns]
+    // address spaces made explicit (LDS and global instructions instead of
+    // flat ones): the strip and the propagator factors live in LDS, the
+    // operator blocks and the state matrices in global memory (blocks above
+    // 16 states)
+    lds_d *sb = (lds_d *)e.strip;   // [128 rows][16 columns]
+    lds_d *dvec = (lds_d *)e.dvec;
+    const glb_d *Kg = (const glb_d *)Kin;
+    glb_d *Kog = (glb_d *)Kout;
     // superstate path, quantum numbers and the common log factor (uniform)
     int s = s0;
     double t = tstart;
@@ -939,9 +995,10 @@ __device__ __attribute__((noinline)) MfmaOut walk_mfma(const MfmaEnv e, const do
     int emax = -2147483647, es_of[8];
     for (int q = 0; q < 8; ++q) es_of[q] = 0;
     for (int q = 0; q * 16 < no; ++q) {
-        for (int idx = l; idx < ns0 * 16; idx += WAVE) {
+        // rows beyond the block stay zero up to the 16-row padding
+        for (int idx = l; idx < (ns0 + 15) / 16 * 256; idx += WAVE) {
             const int r = idx >> 4, i = q * 16 + (idx & 15);
-            sb[idx] = i < no ? Kin[(long)r * no + i] : 0.0;
+            sb[idx] = (r < ns0 && i < no) ? Kg[(long)r * no + i] : 0.0;
         }
         wsync();
         int es = 0, ns = ns0;
@@ -950,36 +1007,35 @@ __device__ __attribute__((noinline)) MfmaOut walk_mfma(const MfmaEnv e, const do
         for (int jj = 0; jj < n; ++jj) {
             const int j = BRA ? n - 1 - jj : jj;
             const double tj = otau[j];
-            const int4 tr = mfma_trans(e, s, ocode[j], BRA);
-            const int nt = tr.y, nkt = (ns + 15) >> 4, ntp = (nt + 15) >> 4;
+            const int ti = mfma_tidx(e, s, ocode[j], BRA);
+            const int4 tr = e.tr[ti];
+            // uniform bounds (scalar loop control, no exec-mask branches)
+            const int nt = __builtin_amdgcn_readfirstlane(tr.y);
+            const int nkt = __builtin_amdgcn_readfirstlane((ns + 15) >> 4);
+            const int ntp = __builtin_amdgcn_readfirstlane((nt + 15) >> 4);
+            const int ldp = 16 * ntp;
             mfma_prop(e, s, BRA ? t - tj : tj - t);
-            const double *P = e.ops + tr.z;
+            const glb_d *P = (const glb_d *)(e.ops16 + __builtin_amdgcn_readfirstlane(e.off16[ti]));
             dbl4 out[8];
 #pragma unroll
             for (int tp = 0; tp < 8; ++tp) out[tp] = dbl4{0.0, 0.0, 0.0, 0.0};
-            for (int kt = 0; kt < nkt; ++kt) {
-#pragma unroll
-                for (int ee = 0; ee < 4; ++ee) {
-                    const int k = 16 * kt + 4 * ee + lk;
-                    const bool kin = k < ns;
-                    const double b = kin ? sb[k * 16 + lr] * e.dvec[k] : 0.0;
-                    const double *Pk = P + (long)k * nt;
-#pragma unroll
-                    for (int tp = 0; tp < 8; ++tp) {
-                        if (tp < ntp) {
-                            const int r = 16 * tp + lr;
-                            const double a = (r < nt && kin) ? Pk[r] : 0.0;
-                            out[tp] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, out[tp], 0, 0, 0);
-                        }
-                    }
-                }
+            switch (ntp) {   // row tiles as a compile-time count: straight-line loads and MFMAs
+            case 1: mfma_kloop<1>(out, P, ldp, nkt, sb, dvec, lr, lk); break;
+            case 2: mfma_kloop<2>(out, P, ldp, nkt, sb, dvec, lr, lk); break;
+            case 3: mfma_kloop<3>(out, P, ldp, nkt, sb, dvec, lr, lk); break;
+            case 4: mfma_kloop<4>(out, P, ldp, nkt, sb, dvec, lr, lk); break;
+            case 5: mfma_kloop<5>(out, P, ldp, nkt, sb, dvec, lr, lk); break;
+            case 6: mfma_kloop<6>(out, P, ldp, nkt, sb, dvec, lr, lk); break;
+            case 7: mfma_kloop<7>(out, P, ldp, nkt, sb, dvec, lr, lk); break;
+            default: mfma_kloop<8>(out, P, ldp, nkt, sb, dvec, lr, lk); break;
             }
+            // rows beyond the target block come out exactly zero (padded A)
             double m = 0.0;
 #pragma unroll
             for (int tp = 0; tp < 8; ++tp)
+                if (tp < ntp)
 #pragma unroll
-                for (int ee = 0; ee < 4; ++ee)
-                    if (16 * tp + lk + 4 * ee < nt) m = fmax(m, fabs(out[tp][ee]));
+                    for (int ee = 0; ee < 4; ++ee) m = fmax(m, fabs(out[tp][ee]));
             int ex = 0;
             if (__ballot(m >= STATE_TINY) == 0ull || __ballot(m > STATE_HUGE) != 0ull) {
                 const double mm = wave_max(m);
@@ -989,11 +1045,12 @@ __device__ __attribute__((noinline)) MfmaOut walk_mfma(const MfmaEnv e, const do
             wsync();   // every lane has read the strip and dvec
 #pragma unroll
             for (int tp = 0; tp < 8; ++tp)
+                if (tp < ntp)
 #pragma unroll
-                for (int ee = 0; ee < 4; ++ee) {
-                    const int r = 16 * tp + lk + 4 * ee;
-                    if (r < nt) sb[r * 16 + lr] = ex ? ldexp(out[tp][ee], -ex) : out[tp][ee];
-                }
+                    for (int ee = 0; ee < 4; ++ee) {
+                        const int r = 16 * tp + lk + 4 * ee;
+                        sb[r * 16 + lr] = ex ? ldexp(out[tp][ee], -ex) : out[tp][ee];
+                    }
             wsync();
             s = tr.x;
             ns = nt;
@@ -1003,7 +1060,7 @@ __device__ __attribute__((noinline)) MfmaOut walk_mfma(const MfmaEnv e, const do
         mfma_prop(e, s, BRA ? t - tend : tend - t);
         for (int idx = l; idx < ns * 16; idx += WAVE) {
             const int r = idx >> 4, i = q * 16 + (idx & 15);
-            if (i < no) Kout[(long)r * no + i] = sb[idx] * e.dvec[r];
+            if (i < no) Kog[(long)r * no + i] = sb[idx] * dvec[r];
         }
         wsync();
         es_of[q] = es;
@@ -1014,7 +1071,7 @@ __device__ __attribute__((noinline)) MfmaOut walk_mfma(const MfmaEnv e, const do
         if (es_of[q] != emax)
             for (int idx = l; idx < nsfin * 16; idx += WAVE) {
                 const int r = idx >> 4, i = q * 16 + (idx & 15);
-                if (i < no) Kout[(long)r * no + i] = ldexp(Kout[(long)r * no + i], es_of[q] - emax);
+                if (i < no) Kog[(long)r * no + i] = ldexp(Kog[(long)r * no + i], es_of[q] - emax);
             }
     wsync();
     o.lg += (double)emax * 0.69314718055994530942;
@@ -1045,6 +1102,7 @@ DEV MfmaEnv mfma_env(const Ctx &c)
 {
     MfmaEnv e;
     e.sz = c.sz, e.off = c.off, e.tr = c.tr, e.ev = c.ev, e.ops = c.opsp, e.dvec = c.dvec, e.strip = c.strip;
+    e.ops16 = c.M.ops16, e.off16 = c.M.opoff16;
     e.egs = c.M.egs, e.nb = c.M.nb;
     return e;
 }

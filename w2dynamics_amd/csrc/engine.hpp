@@ -16,6 +16,12 @@ struct DevModel {
     const int *sizes, *offsets, *connect, *first_state, *state_sst;
     const long *opoff;
     const int *opoff32;                 // the same offsets as int (HBM operator-block table for blocks above 16 states)
+    // blocks above 16 states: every operator block zero-padded to multiples of
+    // 16 rows and columns (column-major, leading dimension 16 ceil(n_t / 16)),
+    // offsets indexed like opoff; the MFMA walks read whole 16 x 4 fragments
+    // without bounds checks
+    const double *ops16;
+    const int *opoff16;
     const double *ops, *evals, *ftau;   // ftau[f][nftau]
     const long *dmoff;                  // block-diagonal density-matrix offsets
     const double *legc;                 // Bonnet coefficients (2l+1)/(l+1), l/(l+1) per l

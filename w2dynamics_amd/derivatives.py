@@ -103,3 +103,62 @@ def score_estimate(sums: dict, score: dict, nb: int, beta: float, ntau: int, mod
         a = np.array(jk[k])
         out[k + "_err"] = np.sqrt((n - 1) / n * ((a - a.mean(0)) ** 2).sum(0))
     return out
+
+
+def _pooled(sums: dict, nb: int, beta: float, ntau: int, model: dict) -> dict:
+    """Sign-ratio estimates of G(tau) and the double occupancy from group sums,
+    each group left out in turn (row 0: all groups; row 1 + i: without group i)."""
+    keys = ("gtau", "dm", "scalars")
+    tot = {k: np.asarray(sums[k], float).sum(0) for k in keys}
+    n = len(sums["scalars"])
+    rows = [tot] + [{k: tot[k] - np.asarray(sums[k], float)[i] for k in keys} for i in range(n)]
+    out = []
+    for r in rows:
+        S = r["scalars"][0]
+        g = r["gtau"] / S * ntau / beta ** 2
+        dm = -r["dm"] / S
+        diag = np.zeros(dm.shape[0])
+        for k, (o, m_) in enumerate(zip(model["offsets"], model["sizes"])):
+            v = model["evecs"][k]
+            diag[o:o + m_] = np.einsum("ij,jk,ik->i", v, dm[o:o + m_, o:o + m_], v)
+        states = np.asarray(model["states"])
+        d = np.array([np.sum(diag * ((states >> b) & 1) * ((states >> (b + nb)) & 1)) for b in range(nb)])
+        out.append(dict(gtau=g.reshape(2, nb, ntau).swapaxes(0, 1), double_occ=d))
+    return out
+
+
+def finite_difference_crn(builder, theta, t: int, delta: float, configs, nchains: int, seed: int,
+                          nmeas: int, taudiffmax: float | None = None, capacity: int = 256,
+                          ngroups: int = 64, device: int = 0) -> dict:
+    """Central difference (O(theta + delta e_t) - O(theta - delta e_t)) / (2 delta)
+    of G(tau) and the double occupancy with common random numbers: both runs
+    start every chain from the same configuration (``configs``, as
+    ``CtqmcGpu.get_config`` returns them, e.g. thermalised at theta) with the
+    same seeds, so that their Markov chains coincide until an acceptance
+    decision differs.  Errors: jackknife over groups of chains applied to the
+    difference (which keeps the correlation of the paired runs)."""
+    from .ctqmc import CtqmcGpu
+    res = []
+    for sgn in (1.0, -1.0):
+        th = np.array(theta, float)
+        th[t] += sgn * delta
+        prob, _ = builder(th)
+        gpu = CtqmcGpu.from_problem(prob, nchains=nchains, seed=seed, capacity=capacity, device=device)
+        if taudiffmax is not None:
+            gpu.set_taudiffmax(taudiffmax)
+        for c, cfg in enumerate(configs[:nchains]):
+            gpu.set_config(c, cfg["taus"], cfg["orbs"], cfg["spins"], cfg["cas"], cfg["outer_sst"])
+        gpu.measure(nmeas)
+        ok = int((gpu.chain_status()["status"] == 0).sum())
+        if ok != nchains:
+            raise RuntimeError("finite_difference_crn: %d of %d chains stopped" % (nchains - ok, nchains))
+        res.append(_pooled(gpu.group_sums(int(ngroups)), prob.nbands, prob.beta, gpu.ntau, gpu.model()))
+        gpu.close()
+    out = {}
+    n = len(res[0]) - 1
+    for k in ("gtau", "double_occ"):
+        vals = np.array([(p[k] - m[k]) / (2 * delta) for p, m in zip(res[0], res[1])])
+        jk = vals[1:]
+        out[k] = vals[0]
+        out[k + "_err"] = np.sqrt((n - 1) / n * ((jk - jk.mean(0)) ** 2).sum(0))
+    return out
