@@ -227,68 +227,108 @@ def write_configs(path, cfg):
         np.asarray(cfg["cas"]).astype("<i4").tofile(f)
 
 
-def cpu_baseline(cfg, ncorr, seed, configs):
+class CpuBaseline:
     """The reference CPU path (oracle/_ref/ref_driver, built from the reference
     sources) on this host's cores: one process per core (seeds NSeed + i, like
     MPI ranks), each started from a thermalised configuration of a GPU chain
-    and measuring a fixed number of sweeps.  Falls back to the C oracle (a
-    single-threaded port) if the reference build is absent."""
-    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
-    drv = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
-    try:
-        ncpu = len(os.sched_getaffinity(0))
-    except AttributeError:
-        ncpu = os.cpu_count() or 1
-    nproc = max(1, min(16, ncpu, len(configs)))
-    nmeas = WORKLOADS[cfg][5]
-    prob, tdm = make_problem(cfg, ncorr, Nmeas=nmeas)
-    if os.path.exists(drv):
+    and measuring a fixed number of sweeps.  The processes start before the
+    GPU run (the reference's model setup takes minutes for 1024 states) and
+    wait on standard input until their configurations are written.  Falls back
+    to the C oracle (a single-threaded port) if the reference build is absent."""
+
+    def __init__(self, cfg, ncorr, seed):
+        self.cfg, self.ncorr, self.seed = cfg, ncorr, seed
+        self.nmeas = WORKLOADS[cfg][5]
+        self.prob, self.tdm = make_problem(cfg, ncorr, Nmeas=self.nmeas)
+        self.procs, self.dirs = [], []
+        drv = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
         try:
+            ncpu = len(os.sched_getaffinity(0))
+        except AttributeError:
+            ncpu = os.cpu_count() or 1
+        # one core stays with the GPU process
+        self.nproc = max(1, min(15, ncpu - 1))
+        if not os.path.exists(drv):
+            return
+        try:
+            sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
             import make_golden as MG
-            dirs = []
-            for i in range(nproc):
+            for i in range(self.nproc):
                 d = tempfile.mkdtemp(prefix="w2dref")
-                MG.write_inputs(d, prob, "runcfg", seed + i, 0, 0)
+                MG.write_inputs(d, self.prob, "runcfg", seed + i, 0, 0)
+                self.dirs.append(d)
+            # one core per process, like one MPI rank per core: no BLAS/OpenMP
+            # threads of their own (many threads per process oversubscribe the
+            # cores and stall the model setup)
+            env = dict(os.environ, OMP_NUM_THREADS="1", MKL_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1")
+            self.procs = [subprocess.Popen([drv, d], stdin=subprocess.PIPE, stdout=subprocess.DEVNULL,
+                                           stderr=subprocess.DEVNULL, env=env) for d in self.dirs]
+        except Exception as e:  # reported, then the port is timed instead
+            print("reference CPU run could not start (%s)" % e, file=sys.stderr)
+            self.procs = []
+
+    def finish(self, configs):
+        if self.procs:
+            try:
+                return self._finish_reference(configs)
+            except Exception as e:  # reported, then fall back to the port
+                print("reference CPU run failed (%s); timing the C oracle instead" % e, file=sys.stderr)
+        return self._port(configs)
+
+    def _finish_reference(self, configs):
+        n = min(len(self.procs), len(configs))
+        for i, (p, d) in enumerate(zip(self.procs, self.dirs)):
+            if i < n:
                 write_configs(os.path.join(d, "configs.bin"), configs[i])
-                dirs.append(d)
-            t0 = time.perf_counter()
-            procs = [subprocess.Popen([drv, d], stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
-                     for d in dirs]
-            rcs = [p.wait() for p in procs]
-            wall = time.perf_counter() - t0
-            if all(rc == 0 for rc in rcs):
-                nb, ntau = prob.nbands, int(prob.qmc()["Ntau"])
-                g, tsim, orders = [], [], []
-                for d in dirs:
-                    raw = np.fromfile(os.path.join(d, "results.bin"), dtype="<f8")
-                    g.append(raw[1:1 + nb * 2 * ntau].reshape((nb, 2, ntau), order="F"))
-                    tw, ts, nop = open(os.path.join(d, "timing.txt")).read().split()
-                    tsim.append(float(ts))
-                    orders.append(int(nop))
-                moves = nproc * nmeas * ncorr
-                return dict(value=moves / wall, unit="moves/s", cores=nproc, kind="reference",
-                            cpu=cpu_model(),
-                            sample="%d processes x %d measured sweeps of %d moves from thermalised "
-                                   "configurations (mean order %.0f operators), %s" % (
-                                       nproc, nmeas, ncorr, np.mean([c["n"] for c in configs[:nproc]]),
-                                       prob.label),
-                            gtau_inv_err2_per_s=gtau_inv_err2({"gtau": np.array(g)}) / wall,
-                            wall_s=wall, sampling_s_max=max(tsim), final_mean_order=float(np.mean(orders)))
-            print("reference runs returned %s" % rcs, file=sys.stderr)
-        except Exception as e:  # reported, then fall back to the port
-            print("reference CPU run failed (%s); timing the C oracle instead" % e, file=sys.stderr)
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    from oracle import OracleChain
-    o = OracleChain.from_problem(prob, seed)
-    o.set_taudiffmax(tdm)
-    c = configs[0]
-    o.set_config(c["taus"], c["orbs"], c["spins"], c["cas"], c["outer_sst"])
-    t0 = time.perf_counter()
-    o.measure(nmeas, ncorr)
-    wall = time.perf_counter() - t0
-    return dict(value=nmeas * ncorr / wall, unit="moves/s", cores=1, kind="port", cpu=cpu_model(),
-                sample="1 chain x %d sweeps of %d moves from a thermalised configuration, %s" % (
-                    nmeas, ncorr, prob.label), wall_s=wall)
+                p.stdin.write(b"go\n")
+            p.stdin.close()
+        t0 = time.perf_counter()
+        rcs = [p.wait() for p in self.procs]
+        wall = time.perf_counter() - t0
+        if not all(rc == 0 for rc in rcs[:n]):
+            raise RuntimeError("reference runs returned %s" % rcs)
+        nb, ntau = self.prob.nbands, int(self.prob.qmc()["Ntau"])
+        g, tsim, orders = [], [], []
+        for d in self.dirs[:n]:
+            raw = np.fromfile(os.path.join(d, "results.bin"), dtype="<f8")
+            g.append(raw[1:1 + nb * 2 * ntau].reshape((nb, 2, ntau), order="F"))
+            tw, ts, nop = open(os.path.join(d, "timing.txt")).read().split()
+            tsim.append(float(ts))
+            orders.append(int(nop))
+        moves = self.nmeas * self.ncorr
+        # the processes sample concurrently: the aggregate rate is the sum of
+        # each one's moves over its own sampling time (the model setup is not
+        # sampling)
+        value = float(sum(moves / t for t in tsim))
+        return dict(value=value, unit="moves/s", cores=n, kind="reference", cpu=cpu_model(),
+                    sample="%d processes x %d measured sweeps of %d moves from thermalised configurations "
+                           "(mean order %.0f operators), %s" % (
+                               n, self.nmeas, self.ncorr, np.mean([c["n"] for c in configs[:n]]),
+                               self.prob.label),
+                    gtau_inv_err2_per_s=gtau_inv_err2({"gtau": np.array(g)}) / max(tsim),
+                    sampling_s=[round(t, 3) for t in tsim], wall_after_release_s=wall,
+                    final_mean_order=float(np.mean(orders)))
+
+    def _port(self, configs):
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        from oracle import OracleChain
+        o = OracleChain.from_problem(self.prob, self.seed)
+        o.set_taudiffmax(self.tdm)
+        c = configs[0]
+        o.set_config(c["taus"], c["orbs"], c["spins"], c["cas"], c["outer_sst"])
+        t0 = time.perf_counter()
+        o.measure(self.nmeas, self.ncorr)
+        wall = time.perf_counter() - t0
+        return dict(value=self.nmeas * self.ncorr / wall, unit="moves/s", cores=1, kind="port", cpu=cpu_model(),
+                    sample="1 chain x %d sweeps of %d moves from a thermalised configuration, %s" % (
+                        self.nmeas, self.ncorr, self.prob.label), wall_s=wall)
+
+    def cancel(self):
+        for p in self.procs:
+            try:
+                p.kill()
+            except OSError:
+                pass
 
 
 def copy_probe(mib: int):
@@ -347,8 +387,15 @@ def main():
     capacity = WORKLOADS[cfg][4]
     seed = args.seed
 
+    # the reference CPU processes set their model up while the GPU runs
+    cpu_run = None if args.no_cpu else CpuBaseline(cfg, args.ncorr, seed)
     print("batched run: %s, %d chains" % (cfg, nchains), file=sys.stderr, flush=True)
-    b = run_gpu(cfg, nchains, capacity, args.warmup, args.steps, args.ncorr, seed, args.therm_s)
+    try:
+        b = run_gpu(cfg, nchains, capacity, args.warmup, args.steps, args.ncorr, seed, args.therm_s)
+    except BaseException:
+        if cpu_run:
+            cpu_run.cancel()
+        raise
     value = b["moves"] / b["wall"]
     inv_err2_s = b["inv_err2"] / b["wall"]
     kernel_s = b["kernel_ms"] / 1e3
@@ -382,9 +429,9 @@ def main():
                      plateau_curve=p["curve"])
 
     cpu = None
-    if not args.no_cpu:
+    if cpu_run:
         print("CPU baseline", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(cfg, args.ncorr, seed, b["configs"])
+        cpu = cpu_run.finish(b["configs"])
 
     st = b["status"]
     line = {

@@ -10,8 +10,9 @@
 !   run   : ctqmc_warmup(NWARM) + ctqmc_measure(1,1) and dump the observables
 !   eval  : set_mc_config for each configuration in <dir>/configs.bin and dump the
 !           trace / determinant / inverse hybridisation matrices
-!   runcfg: set_mc_config with the first configuration of <dir>/configs.bin (a
-!           thermalised state, as ReuseMCConfig does), then as `run`; also
+!   runcfg: after one line on standard input (or its end), set_mc_config with the
+!           first configuration of <dir>/configs.bin (a thermalised state, as
+!           ReuseMCConfig does), then as `run`; also
 !           writes <dir>/timing.txt (time_sim of the sampling and the final order)
 !
 ! run.txt layout (free format, one item per line):
@@ -87,6 +88,10 @@ program ref_driver
    case ('calib')
       call run_calib(trim(dir), nwarm)
    case ('runcfg')
+      ! a caller may start the (slow) model setup early and release the run
+      ! once configs.bin is written: one line on standard input (end of input
+      ! releases it as well)
+      read(*, '(A)', iostat=ios) line
       call start_from_config(trim(dir))
       call ctqmc_warmup(nwarm)
       call ctqmc_measure(1, 1)
