@@ -135,14 +135,18 @@ def thermalise(gpu, ncorr, min_sweeps, chunk, budget_s, tol=0.015):
     return traj
 
 
-def run_gpu(cfg, nchains, capacity, min_sweeps, steps, ncorr, seed, therm_s):
+def run_gpu(cfg, nchains, capacity, min_sweeps, steps, ncorr, seed, therm_s, launch=None, engine=0):
     import torch
     from w2dynamics_amd.ctqmc import CtqmcGpu, check_accumulators
     prob, tdm = make_problem(cfg, ncorr)
     gpu = CtqmcGpu.from_problem(prob, nchains=nchains, seed=seed, capacity=capacity, device=0)
+    gpu.set_launch(**(launch or {}))
+    if engine > 0:
+        gpu.set_engine(engine)
     gpu.set_taudiffmax(tdm)
     chunk = 5 if cfg in ("c3", "c4") else 20
     traj = thermalise(gpu, ncorr, min_sweeps, chunk, therm_s)
+    tuned = gpu.autotune(2 * ncorr) if engine < 0 else None
     order_before = float(gpu.orders().mean())
     print("timed measurement: %d sweeps" % steps, file=sys.stderr, flush=True)
     torch.cuda.synchronize()
@@ -164,7 +168,7 @@ def run_gpu(cfg, nchains, capacity, min_sweeps, steps, ncorr, seed, therm_s):
                order_before=order_before, order_after=order_after, traj=traj,
                status=gpu.chain_status(), nops_bytes=8.0 * gpu.model_ops_size(),
                configs=[gpu.get_config(c) for c in range(min(nchains, 64))], label=prob.label,
-               beta=prob.beta, tdm=tdm)
+               beta=prob.beta, tdm=tdm, engine=gpu.engine(), tuned=tuned)
     gpu.close()
     return out
 
@@ -370,6 +374,10 @@ def main():
     ap.add_argument("--seed", type=int, default=43890)
     ap.add_argument("--therm-s", type=float, default=240.0, help="thermalisation time budget")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graphs", action="store_true", help="plain stream launches instead of one HIP graph per pass")
+    ap.add_argument("--no-order", action="store_true", help="launch chains in index order (no longest-first order)")
+    ap.add_argument("--engine", type=int, default=-1, help="trace engine (padded block size); 0: the model's smallest, "
+                    "-1: autotune")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run 5 device copies of this many MiB (HBM counter calibration)")
@@ -391,7 +399,8 @@ def main():
     cpu_run = None if args.no_cpu else CpuBaseline(cfg, args.ncorr, seed)
     print("batched run: %s, %d chains" % (cfg, nchains), file=sys.stderr, flush=True)
     try:
-        b = run_gpu(cfg, nchains, capacity, args.warmup, args.steps, args.ncorr, seed, args.therm_s)
+        b = run_gpu(cfg, nchains, capacity, args.warmup, args.steps, args.ncorr, seed, args.therm_s,
+                    launch=dict(graphs=not args.no_graphs, order_chains=not args.no_order), engine=args.engine)
     except BaseException:
         if cpu_run:
             cpu_run.cancel()
@@ -453,7 +462,9 @@ def main():
             "mean_order_ops_before": b["order_before"], "mean_order_ops_after": b["order_after"],
             "thermalisation": [[s, round(o, 2), round(t, 1)] for s, o, t in b["traj"]],
             "capacity": st["capacity"], "max_order_per_flavour": int(st["kfmax"].max()),
-            "chains_failed": int((st["status"] != 0).sum())},
+            "chains_failed": int((st["status"] != 0).sum()),
+            "trace_engine": b["engine"][0], "autotune_moves_per_s": b["tuned"],
+            "launch": {"hip_graph": not args.no_graphs, "chains_longest_first": not args.no_order}},
         "gtau_inv_err2_per_s": inv_err2_s,
         "mean_sign": b["mean_sign"],
         "plain_single_chain": plain,

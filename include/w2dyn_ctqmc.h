@@ -243,6 +243,29 @@ int w2d_ctqmc_aborted(const w2d_ctqmc *h, int *aborted);
  * handler raises the abort word of every live handle. */
 int w2d_ctqmc_trap_signals(void);
 
+/* ---- launch machinery (no counterpart in the reference: one process there
+ * runs one chain) ----------------------------------------------------------
+ * graphs != 0: the sweep launches of a warmup / measure pass are captured into
+ * one HIP graph and launched together (default on); order_chains != 0: before
+ * every sweep launch the chains are ordered by expansion order, longest first,
+ * so that a launch's tail is short (default on); moves_per_launch: moves per
+ * chain in one sweep launch (rounded to whole measurement intervals; 0 = the
+ * default of about 4000).  None of them changes any chain's results. */
+int w2d_ctqmc_set_launch(w2d_ctqmc *h, int graphs, int order_chains, int64_t moves_per_launch);
+
+/* Trace engine = padded block size: 1 (scalar blocks), 4 (register walks), 16
+ * (per-lane products) or 128 (FP64 MFMA tiles), at least nm_min, the smallest
+ * the model's quantum-number blocks allow (the default).  Switching re-lays out
+ * the chains (state kept); nm <= 0 selects nm_min. */
+int w2d_ctqmc_set_engine(w2d_ctqmc *h, int nm);
+int w2d_ctqmc_get_engine(const w2d_ctqmc *h, int *nm, int *nm_min);
+
+/* Autotuner: times every admissible engine over `moves` warm-up moves per
+ * chain (after moves/4 untimed ones that rebuild its caches) and keeps the
+ * fastest; rates[4] (may be NULL) receives moves/s for engines 1, 4, 16, 128
+ * (0 where not admissible).  The moves are warm-up moves of the chains. */
+int w2d_ctqmc_autotune(w2d_ctqmc *h, int64_t moves, int *nm, double *rates);
+
 /* ---- parameter derivatives (no counterpart in the reference) ---------------
  * For callers that differentiate observables with respect to model parameters
  * theta (score-function estimators, d<O>/dtheta = <O s> - <O><s> + <dO/dtheta>

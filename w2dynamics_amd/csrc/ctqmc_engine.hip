@@ -45,6 +45,13 @@ struct w2d_ctqmc {
     void *scratch = nullptr;   // device address of the static scratch on this handle's GPU
     bool grow = true;          // capacity doubles on demand (create's capacity >= 0)
     int3 calib{0, 1, 0};       // calibration mode of the sweeps: phase, check period, pairs per flavour
+    // launch machinery: chains ordered longest first before every sweep launch,
+    // the launches of a warmup / measure pass captured into one HIP graph,
+    // moves per chain and launch (0: whole measurement intervals of ~4000)
+    int nm_min = 0;            // smallest trace engine (padded block size) the model allows
+    int *order = nullptr;      // device: chain of each workgroup
+    bool order_chains = true, graphs = true;
+    long long per_launch = 0;
     ~w2d_ctqmc();
 };
 
@@ -332,18 +339,19 @@ int layout_and_alloc(w2d_ctqmc *h, int nbound)
 // Move every chain into a layout of capacity kc and nbound window boundaries,
 // keeping each chain's state (configuration, random stream, inverses,
 // accumulators; boundary caches when the window count stays).
-int relayout(w2d_ctqmc *h, int kc, int nbound)
+int relayout(w2d_ctqmc *h, int kc, int nbound, int nm_copy = 0)
 {
     w2d::ChainLayout B;
     int st = compute_layout(h, nbound, kc, B);
     if (st) return st;
+    if (nm_copy <= 0) nm_copy = h->nm;
     char *nc = nullptr;
     HIPCHK(hipStreamSynchronize(h->stream));
     HIPCHK(hipMalloc(&nc, chains_bytes(h, B)));
     HIPCHK(hipMemsetAsync(nc, 0, chains_bytes(h, B), h->stream));
     const w2d::ChainLayout &A = h->L;
     hipLaunchKernelGGL(w2d::migrate_kernel, dim3(h->nchains), dim3(w2d::WAVE), 0, h->stream, A, B,
-                       (const char *)h->chains, nc, 2 * h->model.nbands, h->nm, (long)(A.stride - A.gtau));
+                       (const char *)h->chains, nc, 2 * h->model.nbands, nm_copy, (long)(A.stride - A.gtau));
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(h->stream));
     HIPCHK(hipFree(h->chains));
@@ -382,21 +390,35 @@ void launch_init(w2d_ctqmc *h)
                        h->chains);
 }
 
+// LDS attribute of the sweep kernels (set outside any stream capture)
+template <int NM>
+void prepare_sweep(w2d_ctqmc *h)
+{
+    allow_lds(w2d::sweep_kernel<NM, true>, h);
+    allow_lds(w2d::sweep_kernel<NM, false>, h);
+}
+
 template <int NM>
 void launch_sweep(w2d_ctqmc *h, long long target, long long ncorr, int only, w2d_move_record *rec,
                   long long rec_base)
 {
-    if (h->dm.score && h->dm.ntheta > 0) {
-        allow_lds(w2d::sweep_kernel<NM, true>, h);
+    const int *order = nullptr;
+    if (h->order_chains && only < 0) {
+        // chains with more operators first (bucket = nop >> shift below 1024)
+        int shift = 0;
+        while ((h->L.cap >> shift) >= w2d::ORDER_BUCKETS) ++shift;
+        hipLaunchKernelGGL(w2d::order_kernel, dim3(1), dim3(1024), 0, h->stream,
+                           reinterpret_cast<const ChainHdr *>(h->chains + h->L.hdrs), h->nchains, shift, h->order);
+        order = h->order;
+    }
+    if (h->dm.score && h->dm.ntheta > 0)
         hipLaunchKernelGGL((w2d::sweep_kernel<NM, true>), dim3(h->nchains), dim3(w2d::WAVE), (unsigned)h->L.lds_bytes,
                            h->stream, h->dm, h->L, h->chains, target, ncorr, only, rec, rec_base,
-                           (const int *)h->abort_dev, h->calib);
-    } else {
-        allow_lds(w2d::sweep_kernel<NM, false>, h);
+                           (const int *)h->abort_dev, h->calib, order);
+    else
         hipLaunchKernelGGL((w2d::sweep_kernel<NM, false>), dim3(h->nchains), dim3(w2d::WAVE), (unsigned)h->L.lds_bytes,
                            h->stream, h->dm, h->L, h->chains, target, ncorr, only, rec, rec_base,
-                           (const int *)h->abort_dev, h->calib);
-    }
+                           (const int *)h->abort_dev, h->calib, order);
 }
 
 template <int NM>
@@ -575,17 +597,47 @@ int run_moves(w2d_ctqmc *h, long long nmoves, long long ncorr)
 {
     h->moved = true;
     h->aborted = 0;
-    long long per = ncorr > 0 ? ncorr * std::max<long long>(1, 4000 / ncorr) : 4000;
+    const long long base = h->per_launch > 0 ? h->per_launch : 4000;
+    long long per = ncorr > 0 ? ncorr * std::max<long long>(1, base / ncorr) : base;
     h->last_launches = 0;
     h->last_ms = 0.0;
     std::vector<ChainHdr> hd;
     long long from = 0;
     for (bool again = true; again;) {
+        DISPATCH(prepare_sweep, h);
+        const bool graph = h->graphs && (nmoves - from + per - 1) / per >= 2;
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ge = nullptr;
+        if (graph) {
+            // the pass's launches (order + sweep per chunk) as one graph
+            HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+            launch_range(h, from, nmoves, ncorr, per, -1, nullptr, 0);
+            hipError_t e = hipStreamEndCapture(h->stream, &g);
+            if (e == hipSuccess) e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+            if (e != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                HIPCHK(e);
+            }
+        }
         HIPCHK(hipEventRecord(h->ev0, h->stream));
-        launch_range(h, from, nmoves, ncorr, per, -1, nullptr, 0);
+        if (graph) {
+            hipError_t e = hipGraphLaunch(ge, h->stream);
+            if (e != hipSuccess) {
+                (void)hipGraphExecDestroy(ge);
+                (void)hipGraphDestroy(g);
+                HIPCHK(e);
+            }
+        } else {
+            launch_range(h, from, nmoves, ncorr, per, -1, nullptr, 0);
+        }
         HIPCHK(hipGetLastError());
         HIPCHK(hipEventRecord(h->ev1, h->stream));
-        HIPCHK(hipEventSynchronize(h->ev1));
+        hipError_t se = hipEventSynchronize(h->ev1);
+        if (graph) {
+            (void)hipGraphExecDestroy(ge);
+            (void)hipGraphDestroy(g);
+        }
+        HIPCHK(se);
         float ms = 0.f;
         HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
         h->last_ms += ms;
@@ -641,6 +693,77 @@ int group_sums_into(w2d_ctqmc *h, int ngroups, void *work, double *gtau, double 
     if (scalars) HIPCHK(hipMemcpy(scalars, scal, (size_t)ngroups * 14 * 8, hipMemcpyDeviceToHost));
     if (nused) HIPCHK(hipMemcpy(nused, used, (size_t)ngroups * 4, hipMemcpyDeviceToHost));
     return W2D_OK;
+}
+
+// Zero-padded copies of the operator blocks for the MFMA walks (engine for
+// blocks above 16 states): rows and columns padded to multiples of 16,
+// column-major with leading dimension 16 ceil(n_t / 16); offsets indexed like
+// the transition records.
+int build_ops16(w2d_ctqmc *h)
+{
+    if (h->dm.ops16) return W2D_OK;
+    const w2d::LocalModel &m = h->model;
+    const int nb = m.nbands;
+    std::vector<double> ops16;
+    std::vector<int> opoff16(m.opoff.size(), -1);
+    for (size_t i = 0; i < m.connect.size(); ++i) {
+        const int ts = m.connect[i];
+        if (ts < 0 || m.opoff[i] < 0) continue;
+        const int src = (int)(i / (4 * (size_t)nb));   // superstate of this record
+        const int nt = m.sizes[ts], ns = m.sizes[src];
+        const int ntp = (nt + 15) / 16 * 16, nsp = (ns + 15) / 16 * 16;
+        opoff16[i] = (int)ops16.size();
+        ops16.resize(ops16.size() + (size_t)ntp * nsp, 0.0);
+        double *dst = ops16.data() + opoff16[i];
+        const double *srcb = m.ops.data() + m.opoff[i];
+        for (int k = 0; k < ns; ++k)
+            for (int r = 0; r < nt; ++r) dst[r + (size_t)k * ntp] = srcb[r + (size_t)k * nt];
+    }
+    int st;
+    const double *o = nullptr;
+    const int *f = nullptr;
+    if ((st = upload(h, ops16, &o)) || (st = upload(h, opoff16, &f))) return st;
+    h->dm.ops16 = o;
+    h->dm.opoff16 = f;
+    return W2D_OK;
+}
+
+// moves of the next pass count from zero on every chain
+int rewind(w2d_ctqmc *h)
+{
+    std::vector<ChainHdr> hd;
+    int st = read_headers(h, hd);
+    if (st) return st;
+    for (auto &x : hd) {
+        x.pos = 0;
+        if (x.halt != w2d::HALT_CAP) x.halt = w2d::HALT_NONE;
+    }
+    return write_headers(h, hd);
+}
+
+// Change the trace engine (padded block size 1, 4, 16 or 128, at least what
+// the model's blocks need) of every chain: their memory is re-laid out and the
+// trace caches are rebuilt by the next moves.
+int switch_engine(w2d_ctqmc *h, int nm)
+{
+    if (nm != 1 && nm != 4 && nm != 16 && nm != 128) return W2D_ERR_ARG;
+    if (nm < h->nm_min) return W2D_ERR_ARG;
+    if (nm == h->nm) return W2D_OK;
+    int st;
+    if (nm > 16 && (st = build_ops16(h))) return st;
+    const int old = h->nm;
+    h->nm = nm;
+    if ((st = relayout(h, h->kcap, h->L.nbound, std::min(old, nm)))) {
+        h->nm = old;
+        return st;
+    }
+    std::vector<ChainHdr> hd;
+    if ((st = read_headers(h, hd))) return st;
+    for (auto &x : hd) {
+        x.cache_pos = -1;
+        x.wvalid = 0;
+    }
+    return write_headers(h, hd);
 }
 
 }  // namespace
@@ -721,11 +844,18 @@ int w2d_ctqmc_create(const char *params, int nbands, int nftau, const double *u_
     else if (m.nmax <= 16) h->nm = 16;
     else if (m.nmax <= 128) h->nm = 128;
     else return W2D_ERR_ARG;
+    h->nm_min = h->nm;
     HIPCHK(hipSetDevice(device));
     HIPCHK(hipDeviceGetAttribute(&h->lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
     HIPCHK(hipStreamCreate(&h->stream));
     HIPCHK(hipEventCreate(&h->ev0));
     HIPCHK(hipEventCreate(&h->ev1));
+    {
+        void *o = nullptr;
+        HIPCHK(hipMalloc(&o, sizeof(int) * (size_t)nchains));
+        h->dev_allocs.push_back(o);
+        h->order = static_cast<int *>(o);
+    }
     HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&h->abort_host), sizeof(int),
                          hipHostMallocMapped | hipHostMallocCoherent));
     *h->abort_host = 0;
@@ -772,24 +902,6 @@ int w2d_ctqmc_create(const char *params, int nbands, int nftau, const double *u_
     std::vector<int> first_state(m.nsst);
     for (int s = 0; s < m.nsst; ++s) first_state[s] = m.states[m.offsets[s]];
     std::vector<int> opoff32(m.opoff.begin(), m.opoff.end());
-    // zero-padded copies of the operator blocks for the MFMA walks (blocks above 16 states)
-    std::vector<double> ops16;
-    std::vector<int> opoff16(m.opoff.size(), -1);
-    if (h->nm > 16) {
-        for (size_t i = 0; i < m.connect.size(); ++i) {
-            const int ts = m.connect[i];
-            if (ts < 0 || m.opoff[i] < 0) continue;
-            const int src = (int)(i / (4 * (size_t)nbands));   // superstate of this record
-            const int nt = m.sizes[ts], ns = m.sizes[src];
-            const int ntp = (nt + 15) / 16 * 16, nsp = (ns + 15) / 16 * 16;
-            opoff16[i] = (int)ops16.size();
-            ops16.resize(ops16.size() + (size_t)ntp * nsp, 0.0);
-            double *dst = ops16.data() + opoff16[i];
-            const double *srcb = m.ops.data() + m.opoff[i];
-            for (int k = 0; k < ns; ++k)
-                for (int r = 0; r < nt; ++r) dst[r + (size_t)k * ntp] = srcb[r + (size_t)k * nt];
-        }
-    }
     std::vector<int4> trans(m.connect.size());
     for (size_t i = 0; i < m.connect.size(); ++i) {
         const int ts = m.connect[i];
@@ -802,8 +914,7 @@ int w2d_ctqmc_create(const char *params, int nbands, int nftau, const double *u_
         (st = upload(h.get(), m.ops, &d.ops)) || (st = upload(h.get(), m.evals, &d.evals)) ||
         (st = upload(h.get(), fdiag, &d.ftau)) || (st = upload(h.get(), dmoff, &d.dmoff)) ||
         (st = upload(h.get(), legc, &d.legc)) || (st = upload(h.get(), trans, &d.trans)) ||
-        (st = upload(h.get(), opoff32, &d.opoff32)) || (st = upload(h.get(), ops16, &d.ops16)) ||
-        (st = upload(h.get(), opoff16, &d.opoff16)))
+        (st = upload(h.get(), opoff32, &d.opoff32)) || (h->nm > 16 && (st = build_ops16(h.get()))))
         return st;
     d.p_shift = P.real("PercentageTauShiftMove");
     d.p_glob = P.real("PercentageGlobalMove");
@@ -996,6 +1107,7 @@ int w2d_ctqmc_moves_log(w2d_ctqmc *h, int chain, int64_t n, w2d_move_record *rec
     if ((st = write_headers(h, hd))) return st;
     w2d_move_record *d = nullptr;
     HIPCHK(hipMalloc(&d, sizeof(w2d_move_record) * per));
+    DISPATCH(prepare_sweep, h);
     for (long long done = 0; done < n && st == W2D_OK;) {
         const long long k = std::min<long long>(per, n - done);
         DISPATCH(launch_sweep, h, done + k, 0, chain, d, done);
@@ -1217,6 +1329,58 @@ int w2d_ctqmc_get_score_sums(w2d_ctqmc *h, int ngroups, double *ssum, double *gt
     return st;
 }
 
+int w2d_ctqmc_set_launch(w2d_ctqmc *h, int graphs, int order_chains, int64_t moves_per_launch)
+{
+    if (!h || moves_per_launch < 0) return W2D_ERR_ARG;
+    h->graphs = graphs != 0;
+    h->order_chains = order_chains != 0;
+    h->per_launch = moves_per_launch;
+    return W2D_OK;
+}
+
+int w2d_ctqmc_set_engine(w2d_ctqmc *h, int nm)
+{
+    if (!h) return W2D_ERR_ARG;
+    return switch_engine(h, nm > 0 ? nm : h->nm_min);
+}
+
+int w2d_ctqmc_get_engine(const w2d_ctqmc *h, int *nm, int *nm_min)
+{
+    if (!h) return W2D_ERR_ARG;
+    if (nm) *nm = h->nm;
+    if (nm_min) *nm_min = h->nm_min;
+    return W2D_OK;
+}
+
+int w2d_ctqmc_autotune(w2d_ctqmc *h, int64_t moves, int *nm, double *rates)
+{
+    if (!h || moves < 1) return W2D_ERR_ARG;
+    const int cand[4] = {1, 4, 16, 128};
+    int best = h->nm;
+    double best_rate = -1.0;
+    for (int i = 0; i < 4; ++i) {
+        if (rates) rates[i] = 0.0;
+        if (cand[i] < h->nm_min) continue;
+        int st = switch_engine(h, cand[i]);
+        if (st == W2D_ERR_CAPACITY) continue;   // this engine's LDS does not fit
+        if (st) return st;
+        // rebuild the caches untimed, then time the engine (positions count
+        // from zero in each pass)
+        if ((st = rewind(h)) || ((st = run_moves(h, std::max<int64_t>(1, moves / 4), 0)) && st != W2D_ERR_PARTIAL))
+            return st;
+        if ((st = rewind(h)) || ((st = run_moves(h, moves, 0)) && st != W2D_ERR_PARTIAL)) return st;
+        const double rate = (double)h->nchains * (double)moves / (h->last_ms * 1e-3);
+        if (rates) rates[i] = rate;
+        if (rate > best_rate) {
+            best_rate = rate;
+            best = cand[i];
+        }
+    }
+    const int st = switch_engine(h, best);
+    if (nm) *nm = h->nm;
+    return st;
+}
+
 int w2d_ctqmc_get_group_sums(w2d_ctqmc *h, int ngroups, double *gtau, double *gleg, double *dm_blocks,
                              double *histo, double *outer, double *scalars, int32_t *nused)
 {
@@ -1308,6 +1472,7 @@ int w2d_ctqmc_calibrate(w2d_ctqmc *h, int list, int phase1pairnum, int phase2tau
     const long long per = 4000;
     long long from = 0;
     h->last_launches = 0;
+    DISPATCH(prepare_sweep, h);
     for (;;) {
         launch_range(h, from, from + per, 0, per, -1, nullptr, 0);
         if (hipGetLastError() != hipSuccess) {

@@ -3429,9 +3429,12 @@ DEV void save_ctx(Ctx &c)
 template <int NM, bool SCORE>
 __global__ void __launch_bounds__(WAVE) sweep_kernel(DevModel M, ChainLayout L, char *chains, long long target,
                                                      long long ncorr, int only, w2d_move_record *rec,
-                                                     long long rec_base, const int *abort_word, int3 calib)
+                                                     long long rec_base, const int *abort_word, int3 calib,
+                                                     const int *order)
 {
-    const int chain = blockIdx.x;
+    // order: the chain each workgroup runs (longest first, so that the
+    // launch's tail is short), or null for workgroup = chain
+    const int chain = order ? order[blockIdx.x] : (int)blockIdx.x;
     if (only >= 0 && chain != only) return;
     extern __shared__ __align__(16) char lds[];
     char *base = chains + (long)chain * L.stride;
@@ -3575,6 +3578,33 @@ __global__ void __launch_bounds__(WAVE) migrate_kernel(ChainLayout A, ChainLayou
 // [g*nc/ng, (g+1)*nc/ng) that have not stopped on an error (header err == 0).
 // Grid: x over elements, y over groups; consecutive threads read consecutive
 // doubles of one chain.
+// Launch order of the chains by expected cost: a counting sort of the chains
+// by expansion order (nop >> shift, clamped to 1023), largest first, in one
+// workgroup (LDS counters; order within a bucket is arbitrary: chains are
+// independent, so their results do not depend on it).  A kernel rather than a
+// host sort, so that it can sit in a captured graph between sweep launches.
+constexpr int ORDER_BUCKETS = 1024;
+__global__ void __launch_bounds__(1024) order_kernel(const ChainHdr *hdrs, int nc, int shift, int *order)
+{
+    __shared__ int cnt[ORDER_BUCKETS];
+    const int t = threadIdx.x;
+    cnt[t] = 0;
+    __syncthreads();
+    auto key = [&](int c) { return ORDER_BUCKETS - 1 - min(hdrs[c].nop >> shift, ORDER_BUCKETS - 1); };
+    for (int c = t; c < nc; c += 1024) atomicAdd(&cnt[key(c)], 1);
+    __syncthreads();
+    if (t == 0) {   // exclusive scan (1024 entries)
+        int run = 0;
+        for (int b = 0; b < ORDER_BUCKETS; ++b) {
+            const int x = cnt[b];
+            cnt[b] = run;
+            run += x;
+        }
+    }
+    __syncthreads();
+    for (int c = t; c < nc; c += 1024) order[atomicAdd(&cnt[key(c)], 1)] = c;
+}
+
 __global__ void __launch_bounds__(256) group_sum_kernel(const char *chains, long stride, long acc_off, long nel,
                                                         int nc, int ng, const ChainHdr *hdrs, double *out)
 {

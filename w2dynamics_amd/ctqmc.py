@@ -92,6 +92,10 @@ def load_library(path: str | None = None):
     lib.w2d_ctqmc_get_group_histograms.argtypes = [P, I, P, P, P, P, P, P]
     lib.w2d_ctqmc_get_steps.argtypes = [P, I, C.POINTER(I), P, P, P]
     lib.w2d_ctqmc_get_group_sums_noalloc.argtypes = [P, I, P, P, P, P, P, P, P]
+    lib.w2d_ctqmc_set_launch.argtypes = [P, I, I, C.c_int64]
+    lib.w2d_ctqmc_set_engine.argtypes = [P, I]
+    lib.w2d_ctqmc_get_engine.argtypes = [P, C.POINTER(I), C.POINTER(I)]
+    lib.w2d_ctqmc_autotune.argtypes = [P, C.c_int64, C.POINTER(I), P]
     lib.w2d_ctqmc_set_derivatives.argtypes = [P, I, P, P, P, I]
     lib.w2d_ctqmc_dlogw.argtypes = [P, I, P]
     lib.w2d_ctqmc_get_score_sums.argtypes = [P, I, P, P, P, P]
@@ -391,6 +395,32 @@ class CtqmcGpu:
         check_accumulators(sums)
         return pooled_estimate(sums, self.nbands, self.beta, self.ntau, self.nleg, self.model())
 
+
+    # -- launch machinery ---------------------------------------------------------
+    def set_launch(self, graphs: bool = True, order_chains: bool = True, moves_per_launch: int = 0):
+        """Sweep launches of a pass captured into one HIP graph; chains ordered
+        longest first before every launch; moves per chain and launch (0: ~4000)."""
+        _check(self.lib.w2d_ctqmc_set_launch(self.h, int(bool(graphs)), int(bool(order_chains)),
+                                             int(moves_per_launch)), "set_launch")
+
+    def engine(self) -> tuple:
+        """(trace engine in use, smallest admissible): padded block sizes 1, 4, 16, 128."""
+        a, b = C.c_int(), C.c_int()
+        _check(self.lib.w2d_ctqmc_get_engine(self.h, C.byref(a), C.byref(b)), "get_engine")
+        return a.value, b.value
+
+    def set_engine(self, nm: int):
+        _check(self.lib.w2d_ctqmc_set_engine(self.h, int(nm)), "set_engine")
+
+    def autotune(self, moves: int = 2000) -> dict:
+        """Time every admissible trace engine over ``moves`` warm-up moves per
+        chain and keep the fastest: {engine: moves/s}, and the choice under 'chosen'."""
+        nm = C.c_int()
+        rates = np.zeros(4)
+        _check(self.lib.w2d_ctqmc_autotune(self.h, int(moves), C.byref(nm), _p(rates)), "autotune")
+        out = {e: float(r) for e, r in zip((1, 4, 16, 128), rates) if r > 0}
+        out["chosen"] = nm.value
+        return out
 
     # -- parameter derivatives (no counterpart in the reference) -----------------
     def set_derivatives(self, deriv, score: bool = True):
