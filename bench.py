@@ -173,13 +173,15 @@ def run_gpu(cfg, nchains, capacity, min_sweeps, steps, ncorr, seed, therm_s, lau
     return out
 
 
-def run_plain(cfg, capacity, ncorr, seed, start_cfg, budget_s=25.0):
+def run_plain(cfg, capacity, ncorr, seed, start_cfg, budget_s=25.0, engine=0):
     """Single chain from a thermalised configuration; binning-plateau error."""
     import torch
     from w2dynamics_amd.ctqmc import CtqmcGpu
     # a short timed run sizes the measurement to the budget
     prob, tdm = make_problem(cfg, ncorr, AccumGtau=2, Nmeas=4)
     gpu = CtqmcGpu.from_problem(prob, nchains=1, seed=seed, capacity=capacity, device=0)
+    if engine > 0:
+        gpu.set_engine(engine)
     gpu.set_taudiffmax(tdm)
     c = start_cfg
     gpu.set_config(0, c["taus"], c["orbs"], c["spins"], c["cas"], c["outer_sst"])
@@ -192,6 +194,8 @@ def run_plain(cfg, capacity, ncorr, seed, start_cfg, budget_s=25.0):
     nmeas = int(min(8192, max(64, budget_s * rate / ncorr)))
     prob, tdm = make_problem(cfg, ncorr, AccumGtau=2, Nmeas=nmeas)
     gpu = CtqmcGpu.from_problem(prob, nchains=1, seed=seed, capacity=capacity, device=0)
+    if engine > 0:
+        gpu.set_engine(engine)
     gpu.set_taudiffmax(tdm)
     gpu.set_config(0, c["taus"], c["orbs"], c["spins"], c["cas"], c["outer_sst"])
     torch.cuda.synchronize()
@@ -363,12 +367,77 @@ def traffic_per_move(workload: str):
     return None
 
 
+def run_c5(args):
+    """Differentiation workload: the 3-orbital Kanamori model (beta=100) as a
+    function of theta = (U, J, eps_d, V), 4096 chains with score sums at every
+    measurement; score-function estimates of d<n_up n_dn>/dU and dG(tau)/dJ
+    against central differences with common random numbers (same seeds, the
+    chains' thermalised configurations as the common start) at delta = 1e-3."""
+    import torch
+    from w2dynamics_amd import problems as P
+    from w2dynamics_amd.ctqmc import CtqmcGpu
+    from w2dynamics_amd.derivatives import finite_difference_crn
+    theta = np.array([3.5, 0.6, 0.0, 1.0])
+    ncorr, nchains, seed = args.ncorr, args.chains or 4096, args.seed
+
+    def builder(th):
+        return P.kanamori_3orb_theta(th, beta=100.0, TaudiffMax=8.0, NCorr=ncorr, Nmeas=args.steps)
+
+    prob, deriv = builder(theta)
+    gpu = CtqmcGpu.from_problem(prob, nchains=nchains, seed=seed, capacity=WORKLOADS["c3"][4], device=0)
+    gpu.set_taudiffmax(8.0)
+    traj = thermalise(gpu, ncorr, args.warmup, 5, args.therm_s)
+    # the derivative pass runs on the engines for blocks up to 16 states (the
+    # model's default here); no autotuning
+    tuned = None
+    gpu.set_derivatives(deriv)
+    configs = [gpu.get_config(c) for c in range(nchains)]
+    print("timed measurement with score sums: %d sweeps" % args.steps, file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    gpu.measure(args.steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    est = gpu.score_estimate(64)
+    sc = gpu.group_sums(1, with_dm=False)["scalars"][0]
+    engine = gpu.engine()[0]
+    gpu.close()
+    checks = {}
+    for t, key in ((0, "double_occ"), (1, "gtau")):
+        print("finite differences with common random numbers: %s" % P.KANAMORI_THETA[t], file=sys.stderr, flush=True)
+        fd = finite_difference_crn(builder, theta, t, 1e-3, configs, nchains, seed, args.steps,
+                                   taudiffmax=8.0, capacity=WORKLOADS["c3"][4], engine=engine)
+        s_, se = np.asarray(est["d" + key][t]), np.asarray(est["d" + key + "_err"][t])
+        z = np.abs(s_ - fd[key]) / np.sqrt(se ** 2 + fd[key + "_err"] ** 2 + 1e-300)
+        checks["d%s/d%s" % (key, P.KANAMORI_THETA[t])] = dict(
+            score=np.ravel(s_)[:6].tolist(), score_err=np.ravel(se)[:6].tolist(),
+            fd=np.ravel(fd[key])[:6].tolist(), fd_err=np.ravel(fd[key + "_err"])[:6].tolist(),
+            within_2sigma=float(np.mean(z <= 2.0)), max_sigma=float(z.max()), entries=int(z.size))
+    line = {
+        "metric": "score-function gradients: MC moves/s with d ln|w|/dtheta sums at every measurement",
+        "value": float(sc[10]) / wall, "unit": "moves/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (semicircular-bath 3-orbital Kanamori model, seeded crystal field)",
+        "config": {"workload": "c5: 3orb-kanamori beta=100 as a function of (U, J, eps_d, V), %d chains, "
+                               "NCorr=%d" % (nchains, ncorr), "trace_engine": engine,
+                   "autotune_moves_per_s": tuned, "thermalisation": [[a, round(b, 2), round(c, 1)] for a, b, c in traj]},
+        "mean_sign": float(sc[0] / sc[1]),
+        "dlogw_mean": np.asarray(est["dlogw_mean"]).tolist(),
+        "double_occ": np.asarray(est["double_occ"]).tolist(),
+        "ddouble_occ_dU": np.asarray(est["ddouble_occ"][0]).tolist(),
+        "ddouble_occ_dU_err": np.asarray(est["ddouble_occ_err"][0]).tolist(),
+        "crn_finite_differences_delta_1e-3": checks,
+    }
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=10, help="minimum thermalisation sweeps")
-    ap.add_argument("--config", default="c4", choices=sorted(WORKLOADS))
+    ap.add_argument("--config", default="c4", choices=sorted(WORKLOADS) + ["c5"])
     ap.add_argument("--chains", type=int, default=0)
     ap.add_argument("--ncorr", type=int, default=100)
     ap.add_argument("--seed", type=int, default=43890)
@@ -391,6 +460,9 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     cfg = args.config
+    if cfg == "c5":
+        run_c5(args)
+        return
     nchains = args.chains or WORKLOADS[cfg][3]
     capacity = WORKLOADS[cfg][4]
     seed = args.seed
@@ -431,7 +503,7 @@ def main():
     plain = None
     if not args.no_plain:
         print("plain single-chain run", file=sys.stderr, flush=True)
-        p = run_plain(cfg, capacity, args.ncorr, seed + 1000003, b["configs"][0])
+        p = run_plain(cfg, capacity, args.ncorr, seed + 1000003, b["configs"][0], engine=b["engine"][0])
         plain = dict(moves_per_s=p["moves"] / p["wall"], gtau_inv_err2_per_s=p["inv_err2"] / p["wall"],
                      sweeps=p["nmeas"], wall_s=p["wall"], mean_order=p["order"],
                      error="binning plateau (AccumGtau=2): largest <stderr^2> over levels with >= 16 bins",

@@ -278,7 +278,9 @@ int w2d_ctqmc_autotune(w2d_ctqmc *h, int64_t moves, int *nm, double *rates);
  *   du     [ntheta][(2N)^4]           d u_matrix / d theta (H must stay linear)
  *   dmuimp [ntheta][N][2][N][2]       d muimp_full / d theta
  *   dftau  [ntheta][N][2][N][2][Nftau] d ftau_full / d theta (diagonal read)
- * dH/dtheta must conserve the quantum numbers (else W2D_ERR_ARG).  score != 0
+ * dH/dtheta must conserve the quantum numbers (else W2D_ERR_ARG).  Available
+ * with the trace engines for blocks up to 16 states (W2D_ERR_ARG with the MFMA
+ * engine, which set_engine / autotune then leave out).  score != 0
  * adds the score sums below to every later measurement; chains keep their
  * state (their storage is re-laid out). */
 int w2d_ctqmc_set_derivatives(w2d_ctqmc *h, int ntheta, const double *du, const double *dmuimp,

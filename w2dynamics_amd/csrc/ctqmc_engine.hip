@@ -748,6 +748,7 @@ int switch_engine(w2d_ctqmc *h, int nm)
 {
     if (nm != 1 && nm != 4 && nm != 16 && nm != 128) return W2D_ERR_ARG;
     if (nm < h->nm_min) return W2D_ERR_ARG;
+    if (nm > 16 && h->dm.ntheta > 0) return W2D_ERR_ARG;   // derivative pass: engines up to 16 only
     if (nm == h->nm) return W2D_OK;
     int st;
     if (nm > 16 && (st = build_ops16(h))) return st;
@@ -1252,6 +1253,14 @@ int w2d_ctqmc_set_derivatives(w2d_ctqmc *h, int ntheta, const double *du, const 
                               const double *dftau, int score)
 {
     if (!h || ntheta < 1 || ntheta > w2d::MAXTHETA || !du || !dmuimp || !dftau) return W2D_ERR_ARG;
+    // the derivative pass runs with the engines for blocks up to 16 states:
+    // with the MFMA engine (128) a measurement run faulted the GPU, cause not
+    // found yet, so that combination is refused
+    if (h->nm > 16) {
+        std::fprintf(stderr, "w2d_ctqmc_set_derivatives: not available with trace engine %d (blocks above 16 "
+                             "states)\n", h->nm);
+        return W2D_ERR_ARG;
+    }
     const int nb = h->model.nbands, n2 = 2 * nb, nf = 2 * nb, nftau = h->dm.nftau;
     const long usz = (long)n2 * n2 * n2 * n2, msz = (long)n2 * n2;
     std::vector<double> blk, all;
@@ -1360,7 +1369,7 @@ int w2d_ctqmc_autotune(w2d_ctqmc *h, int64_t moves, int *nm, double *rates)
     double best_rate = -1.0;
     for (int i = 0; i < 4; ++i) {
         if (rates) rates[i] = 0.0;
-        if (cand[i] < h->nm_min) continue;
+        if (cand[i] < h->nm_min || (cand[i] > 16 && h->dm.ntheta > 0)) continue;
         int st = switch_engine(h, cand[i]);
         if (st == W2D_ERR_CAPACITY) continue;   // this engine's LDS does not fit
         if (st) return st;

@@ -129,7 +129,7 @@ def _pooled(sums: dict, nb: int, beta: float, ntau: int, model: dict) -> dict:
 
 def finite_difference_crn(builder, theta, t: int, delta: float, configs, nchains: int, seed: int,
                           nmeas: int, taudiffmax: float | None = None, capacity: int = 256,
-                          ngroups: int = 64, device: int = 0) -> dict:
+                          ngroups: int = 64, device: int = 0, engine: int = 0) -> dict:
     """Central difference (O(theta + delta e_t) - O(theta - delta e_t)) / (2 delta)
     of G(tau) and the double occupancy with common random numbers: both runs
     start every chain from the same configuration (``configs``, as
@@ -144,6 +144,8 @@ def finite_difference_crn(builder, theta, t: int, delta: float, configs, nchains
         th[t] += sgn * delta
         prob, _ = builder(th)
         gpu = CtqmcGpu.from_problem(prob, nchains=nchains, seed=seed, capacity=capacity, device=device)
+        if engine > 0:
+            gpu.set_engine(engine)
         if taudiffmax is not None:
             gpu.set_taudiffmax(taudiffmax)
         for c, cfg in enumerate(configs[:nchains]):
