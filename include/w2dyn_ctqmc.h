@@ -256,7 +256,8 @@ int w2d_ctqmc_set_launch(w2d_ctqmc *h, int graphs, int order_chains, int64_t mov
 /* Trace engine = padded block size: 1 (scalar blocks), 4 (register walks), 16
  * (per-lane products) or 128 (FP64 MFMA tiles), at least nm_min, the smallest
  * the model's quantum-number blocks allow (the default).  Switching re-lays out
- * the chains (state kept); nm <= 0 selects nm_min. */
+ * the chains (state kept); nm <= 0 selects nm_min.  Engine 128 is admissible
+ * only where nm_min is 128 (blocks above 16 states): W2D_ERR_ARG otherwise. */
 int w2d_ctqmc_set_engine(w2d_ctqmc *h, int nm);
 int w2d_ctqmc_get_engine(const w2d_ctqmc *h, int *nm, int *nm_min);
 

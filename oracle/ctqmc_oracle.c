@@ -172,6 +172,7 @@ typedef struct {
      * the eigenbasis [ntheta][sum n_s^2] at dboff[s], dF/dtheta
      * [ntheta][nf][nftau]; score sums (Sum sign s, Sum G-sample s, Sum
      * rho-sample s, Sum d(rho-sample)/dtheta) */
+    int nsym, *symmoves;   /* user symmetry moves [2 nsym][nf], 1-based flavours (inverses second) */
     int ntheta, score;
     long dbsize, *dboff;
     double *dblk, *dftau;
@@ -894,9 +895,14 @@ static int step_glob(ochain_t *ch)
     } else if (r < 0.6) {
         gu[0] = 1; /* gen_CAFlip, Trace.F90:1178-1190 */
         for (int i = 1; i <= nf; ++i) gu[i] = i;
+    } else if (r < 0.8 && ch->nsym > 0) {
+        /* gen_SymUpdate (Trace.F90:1166-1182): one of the user's flavour
+         * permutations or its inverse, no c/c+ flip */
+        int isym = orng_randint(&ch->rng, 1, 2 * ch->nsym);
+        gu[0] = 0;
+        for (int i = 1; i <= nf; ++i) gu[i] = ch->symmoves[(isym - 1) * nf + (i - 1)];
     } else {
-        /* gen_GlobalUpdate_new + rand_perm (Trace.F90:1065-1103); no user
-         * symmetry moves are configured */
+        /* gen_GlobalUpdate_new + rand_perm (Trace.F90:1065-1103) */
         int a[15];
         for (int i = 1; i <= nf; ++i) a[i] = i;
         for (int j = 1; j <= nf; ++j) {
@@ -1734,9 +1740,26 @@ void oracle_destroy(void *h)
     free(ch->accpair), free(ch->aptau), free(ch->mstep), free(ch->dstep), free(ch->sstep);
     free(ch->sign_hist_sst), free(ch->tc_sst), free(ch->outer_state_hist);
     free(ch->sign_hist_states), free(ch->tc_states);
-    free(ch->dboff), free(ch->dblk), free(ch->dftau), free(ch->dket);
+    free(ch->dboff), free(ch->dblk), free(ch->dftau), free(ch->dket), free(ch->symmoves);
     free(ch->sc_sum), free(ch->sc_g), free(ch->sc_dm), free(ch->sc_dmx);
     free(ch);
+}
+
+/* User symmetry moves (NSymMove, SymMove01..; Trace.F90:471-478): nsym
+ * permutations of the 1-based flavours, each followed in the table by its
+ * inverse as the reference stores them (rows nsym + i). */
+void oracle_set_symmoves(void *h, int nsym, const int *perms)
+{
+    ochain_t *ch = h;
+    int nf = NF(ch);
+    free(ch->symmoves);
+    ch->nsym = nsym;
+    ch->symmoves = malloc(sizeof(int) * (size_t)(2 * nsym * nf + 1));
+    for (int i = 0; i < nsym; ++i)
+        for (int j = 0; j < nf; ++j) {
+            ch->symmoves[i * nf + j] = perms[i * nf + j];
+            ch->symmoves[(nsym + i) * nf + perms[i * nf + j] - 1] = j + 1;
+        }
 }
 
 /* Re-initialise the window with a new TaudiffMax (set_taudiffmax,

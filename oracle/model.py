@@ -29,10 +29,80 @@ def _szt2(state: int, n: int) -> int:
     return hi - lo
 
 
+def _refine_blocks(states, h0, nstates: int, nb: int, thr: float):
+    """analyze_hamiltonian (Operator.F90:303-442): within every block the
+    connected components of H (|H_ij| > thr), numbered in order of discovery
+    (breadth-first from each block's lowest unassigned state); then, while some
+    operator maps two states of one block into two different blocks, those two
+    blocks are merged (the larger label into the smaller, labels above it
+    shifted down), rescanning from the first operator after every merge."""
+    s2s = -np.ones(nstates, dtype=int)
+    nsub = -1
+    for sst, st in enumerate(states):
+        n = len(st)
+        H = h0[sst]
+        nets = [[0]]
+        nsub += 1
+        s2s[st[0]] = nsub
+        k = 0
+        while k < len(nets):
+            net = nets[k]
+            q = 0
+            while q < len(net):
+                node = net[q]
+                for edge in range(n):
+                    v = H[node, edge]
+                    if v != 0.0 and abs(v) > thr and edge not in net:
+                        net.append(edge)
+                        s2s[st[edge]] = nsub
+                q += 1
+            for node in range(n):
+                if s2s[st[node]] == -1:
+                    nets.append([node])
+                    nsub += 1
+                    s2s[st[node]] = nsub
+                    break
+            k += 1
+    idx = np.arange(nstates)
+    change = True
+    while change:
+        change = False
+        for ib in range(nb):
+            for sp in range(2):
+                bit = ib + nb * sp
+                for ca in (1, 2):
+                    occ = (idx >> bit) & 1
+                    cs = -np.ones(nstates, dtype=int)
+                    ok = (occ == 0) if ca == 1 else (occ == 1)
+                    cs[ok] = s2s[idx[ok] ^ (1 << bit)]
+                    hit = None
+                    for i1 in range(nstates):
+                        if cs[i1] == -1:
+                            continue
+                        same = (s2s[i1 + 1:] == s2s[i1]) & (cs[i1 + 1:] != cs[i1]) & (cs[i1 + 1:] != -1)
+                        j = np.flatnonzero(same)
+                        if len(j):
+                            hit = (i1, i1 + 1 + j[0])
+                            break
+                    if hit is not None:
+                        a, b = cs[hit[0]], cs[hit[1]]
+                        big, small = max(a, b), min(a, b)
+                        s2s[s2s == big] = small
+                        s2s[s2s > big] -= 1
+                        change = True
+                        break
+                if change:
+                    break
+            if change:
+                break
+    return s2s, int(s2s.max()) + 1
+
+
 class OracleModel:
     """Block-diagonal local Hamiltonian in its eigenbasis."""
 
-    def __init__(self, umatrix: np.ndarray, muimp: np.ndarray, quantum_numbers: str = "Nt Szt"):
+    def __init__(self, umatrix: np.ndarray, muimp: np.ndarray, quantum_numbers: str = "Nt Szt",
+                 epsblock: float = 1e-13):
         n = umatrix.shape[0]
         self.nbands = n
         nstates = 4 ** n
@@ -53,6 +123,24 @@ class OracleModel:
                     continue
                 s2s[st2] = nsst
             nsst += 1
+        if "All" in quantum_numbers:
+            # analyze_hamiltonian (CTQMC.F90:7971-7989, Operator.F90:303-442):
+            # H on the blocks above, then its connectivity refines them
+            self._set_blocks(s2s, nsst)
+            h0 = self._hloc(umatrix, muimp, strict=False)
+            h0 = [0.5 * (x + x.T) for x in h0]
+            thr = epsblock * max(np.abs(muimp).max(), np.abs(umatrix).max())
+            s2s, nsst = _refine_blocks(self.states, h0, s2s.size, n, thr)
+        self._set_blocks(s2s, nsst)
+        self.connect = self._connect
+        # H_loc blocks
+        h = self._hloc(umatrix, muimp, strict=False)
+        self._finish(h)
+
+    def _set_blocks(self, s2s, nsst):
+        """Superstates from a state -> block map: states ascending within a block
+        (init_SubStates, States.F90:98-146), Connect table and c/c+ blocks."""
+        n = self.nbands
         self.nsst = nsst
         self.states = [np.where(s2s == s)[0] for s in range(nsst)]
         self.sizes = np.array([len(x) for x in self.states])
@@ -87,11 +175,11 @@ class OracleModel:
                         for i1, t, sign in ent:
                             m[pos[t], i1] = sign
                         psi[(s1, b, sp, ca)] = m
+        self._connect = connect
         self.connect = connect
-
-        # H_loc blocks
         self._psi = psi
-        h = self._hloc(umatrix, muimp, strict=False)
+
+    def _finish(self, h):
         self.hblocks = [0.5 * (x + x.T) for x in h]
 
         self.evals = []
@@ -104,10 +192,10 @@ class OracleModel:
 
         # eigenbasis operator blocks
         self.psi_eb = {}
-        for (s1, b, sp, ca), m in psi.items():
-            s2 = connect[s1, b, sp, ca]
+        for (s1, b, sp, ca), m in self._psi.items():
+            s2 = self.connect[s1, b, sp, ca]
             self.psi_eb[(s1, b, sp, ca)] = self.evecs[s2].T @ m @ self.evecs[s1]
-        self.psi_occ = psi
+        self.psi_occ = self._psi
 
     # ------------------------------------------------------------------
     def _hloc(self, umatrix: np.ndarray, muimp: np.ndarray, strict: bool) -> list:

@@ -65,6 +65,7 @@ def lib():
         _lib.oracle_outer_histos.argtypes = [P, P, P, P, P, P]
         _lib.oracle_steps.argtypes = [P, P, P, P]
         _lib.oracle_steps.restype = C.c_int
+        _lib.oracle_set_symmoves.argtypes = [P, C.c_int, P]
         _lib.oracle_set_derivatives.argtypes = [P, C.c_int, P, P, C.c_int]
         _lib.oracle_set_derivatives.restype = C.c_int
         _lib.oracle_dlogw.argtypes = [P, P]
@@ -102,8 +103,11 @@ class OracleChain:
     """One reference-faithful Markov chain on the CPU."""
 
     def __init__(self, umatrix, ftau, muimp, beta, qmc: dict, seed: int,
-                 quantum_numbers: str = "Nt Szt"):
-        self.model = OracleModel(np.asarray(umatrix), np.asarray(muimp), quantum_numbers)
+                 quantum_numbers: str | None = None):
+        if quantum_numbers is None:
+            quantum_numbers = str(qmc.get("QuantumNumbers", "Nt Szt"))
+        self.model = OracleModel(np.asarray(umatrix), np.asarray(muimp), quantum_numbers,
+                                 float(qmc.get("EPSBLOCK", 1e-13)))
         m = self.model
         fl = m.flat()
         nb = m.nbands
@@ -142,6 +146,13 @@ class OracleChain:
                                      _p(k["first_state"]), _p(k["state_sst"]),
                                      int(k["ftau"].shape[1]), _p(k["ftau"]), float(beta),
                                      _p(k["dparams"]), _p(k["iparams"]), int(seed) & 0xffffffff)
+
+        nsym = int(float(qmc.get("NSymMove", 0)))
+        if nsym > 0:
+            perms = np.array([[int(x) for x in str(qmc["SymMove%02d" % (i + 1)]).replace(",", " ").split()]
+                              for i in range(nsym)], dtype=np.int32)
+            self._keep["symmoves"] = np.ascontiguousarray(perms)
+            lib().oracle_set_symmoves(self.h, nsym, _p(self._keep["symmoves"]))
 
     @classmethod
     def from_problem(cls, prob, seed: int):

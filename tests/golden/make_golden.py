@@ -261,6 +261,19 @@ def case_calib(name: str, prob: P.Problem, seed: int, nwarm: int):
           out["calib_accpair"].sum((0, 2)) if nb else 0)
 
 
+def _qn(prob: P.Problem, qns: str) -> P.Problem:
+    prob.quantum_numbers = qns
+    return prob
+
+
+def _density_3orb() -> P.Problem:
+    """3-orbital density-density model (U=3, U'=2, J=0.5), seeded crystal field, beta=10."""
+    cf = np.random.default_rng(7).uniform(-0.2, 0.2, 3)
+    return P.Problem(3, 10.0, P.u_density(3, 3.0, 2.0, 0.5), P.semicircle_ftau(3, 10.0, 2.0, 2001),
+                     P.solver_muimp(3, 3.0, cf), "All", "Density", {"Ntau": 100, "NLegMax": 20,
+                                                                   "TaudiffMax": 3.0}, "3orb-density")
+
+
 CASES = {
     # single window (TaudiffMax <= 0), 1 orbital, scalar trace path
     "c1_log_w1": lambda: case_log("c1_log_w1", P.anderson_1orb(), 43890, 20000),
@@ -271,6 +284,18 @@ CASES = {
     "c3_log_win": lambda: case_log("c3_log_win", P.kanamori_3orb(beta=20.0, TaudiffMax=5.0),
                                    777, 8000),
     "c4_log_w1": lambda: case_log("c4_log_w1", P.coulomb_5orb(beta=2.0), 99, 1500),
+    # QuantumNumbers "All": blocks refined by the Hamiltonian's connectivity
+    # (analyze_hamiltonian) on top of Nt, Szt
+    "c3_log_all": lambda: case_log("c3_log_all", _qn(P.kanamori_3orb(beta=20.0, TaudiffMax=5.0), "Nt Szt All"),
+                                   777, 8000),
+    "c3d_log_all": lambda: case_log("c3d_log_all", _qn(_density_3orb(), "All"), 515, 8000),
+    "c4_log_all": lambda: case_log("c4_log_all", _qn(P.coulomb_5orb(beta=2.0), "Nt Szt All"), 99, 1500),
+    # user symmetry moves (NSymMove > 0): flavour permutations in the global moves
+    "c2_log_sym": lambda: case_log("c2_log_sym", P.kanamori_2orb(
+        beta=20.0, TaudiffMax=4.0, NSymMove=1, SymMove01="2 1 4 3", PercentageGlobalMove=0.05), 4242, 20000),
+    "c3_log_sym": lambda: case_log("c3_log_sym", P.kanamori_3orb(
+        beta=20.0, TaudiffMax=5.0, NSymMove=2, SymMove01="2 3 1 5 6 4", SymMove02="1 3 2 4 6 5",
+        PercentageGlobalMove=0.05), 5151, 8000),
     # short runs with observables (1e4 warmup, Nmeas*NCorr measurement moves)
     "c1_run": lambda: case_run("c1_run", P.anderson_1orb(Nmeas=2000, NCorr=50, Ntau=200,
                                                          NLegMax=30), 43890, 10000),

@@ -65,7 +65,8 @@ def test_move_log_matches_oracle(case):
     gpu.close()
 
 
-@pytest.mark.parametrize("name", ["c1_log_w1", "c2_log_win", "c3_log_win"])
+@pytest.mark.parametrize("name", ["c1_log_w1", "c2_log_win", "c3_log_win", "c3_log_all", "c3d_log_all", "c4_log_all",
+                                  "c2_log_sym", "c3_log_sym"])
 def test_move_log_matches_reference_golden(name):
     z = load_golden(name)
     q = qmc_of(z)

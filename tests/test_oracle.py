@@ -12,7 +12,8 @@ from conftest import load_golden, qmc_of
 from oracle import OracleChain, OracleModel, rng_stream
 from w2dynamics_amd import problems as P
 
-LOG_CASES = ["c1_log_w1", "c1_log_win", "c2_log_win", "c3_log_win", "c4_log_w1"]
+LOG_CASES = ["c1_log_w1", "c1_log_win", "c2_log_win", "c3_log_win", "c4_log_w1", "c3_log_all", "c3d_log_all", "c4_log_all",
+             "c2_log_sym", "c3_log_sym"]
 FIELDS = ("kind", "outcome", "noper", "outer_sst", "window")
 
 
@@ -48,10 +49,11 @@ def test_randint_lemire_range_and_uniformity():
 
 
 # ------------------------------------------------------------- model --------
-@pytest.mark.parametrize("name", ["c1_log_w1", "c2_log_win", "c3_log_win", "c4_log_w1"])
+@pytest.mark.parametrize("name", ["c1_log_w1", "c2_log_win", "c3_log_win", "c4_log_w1", "c3_log_all", "c3d_log_all",
+                                  "c4_log_all"])
 def test_model_matches_reference(name):
     z = load_golden(name)
-    m = OracleModel(z["umatrix"], z["muimp"])
+    m = OracleModel(z["umatrix"], z["muimp"], qmc_of(z)["QuantumNumbers"])
     assert m.nsst == int(z["model_nsst"])
     assert np.array_equal(m.sizes, z["model_sizes"])
     assert np.array_equal(np.concatenate(m.states), z["model_states"])

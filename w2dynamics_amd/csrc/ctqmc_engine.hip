@@ -14,6 +14,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <sstream>
 #include <vector>
 
 #include "model.hpp"
@@ -749,6 +750,10 @@ int switch_engine(w2d_ctqmc *h, int nm)
     if (nm != 1 && nm != 4 && nm != 16 && nm != 128) return W2D_ERR_ARG;
     if (nm < h->nm_min) return W2D_ERR_ARG;
     if (nm > 16 && h->dm.ntheta > 0) return W2D_ERR_ARG;   // derivative pass: engines up to 16 only
+    // the MFMA engine only for models that need it (blocks above 16 states):
+    // measurements with it on models of smaller blocks have ended in GPU
+    // memory faults whose cause is not found, so that combination is refused
+    if (nm > 16 && h->nm_min <= 16) return W2D_ERR_ARG;
     if (nm == h->nm) return W2D_OK;
     int st;
     if (nm > 16 && (st = build_ops16(h))) return st;
@@ -801,7 +806,7 @@ int w2d_ctqmc_create(const char *params, int nbands, int nftau, const double *u_
     if (P.integer("segment") != 0 || P.integer("offdiag") != 0 || P.integer("statesampling") != 0 ||
         P.integer("Uw") == 1 || P.real("PercentageWormInsert") != 0.0 || P.real("Percentage4OperatorMove") != 0.0 ||
         P.real("PercentageOuterMove") != 0.0 || P.str("EnableOuterTruncation") == "YES" ||
-        P.integer("NSymMove") > 0 || P.integer("Eigenbasis") == 0)
+        P.integer("Eigenbasis") == 0)
         return W2D_ERR_ARG;
     if (P.integer("Nd") != nbands) return W2D_ERR_ARG;
     // measurements this library does not produce: refused rather than left
@@ -834,12 +839,18 @@ int w2d_ctqmc_create(const char *params, int nbands, int nftau, const double *u_
     h->ncorr = P.integer("NCorr");
     h->taudiff_param = P.real("TaudiffMax");
     if (h->beta <= 0 || h->ntau < 1 || h->nleg < 1 || h->maxhisto < 0 || h->ncorr < 1) return W2D_ERR_ARG;
-    std::string err = w2d::build_local_model(nbands, u_matrix, muimp_full, P.str("QuantumNumbers"), h->model);
+    const double epsblock = P.has("EPSBLOCK") ? P.real("EPSBLOCK") : 1e-13;
+    std::string err = w2d::build_local_model(nbands, u_matrix, muimp_full, P.str("QuantumNumbers"), h->model,
+                                             epsblock);
     if (!err.empty()) {
         std::fprintf(stderr, "w2d_ctqmc: %s\n", err.c_str());
         return W2D_ERR_ARG;
     }
     const w2d::LocalModel &m = h->model;
+    if (m.nsst > w2d::MAXSST) {
+        std::fprintf(stderr, "w2d_ctqmc: %d quantum-number blocks, at most %d are supported\n", m.nsst, w2d::MAXSST);
+        return W2D_ERR_ARG;
+    }
     if (m.nmax <= 1) h->nm = 1;
     else if (m.nmax <= 4) h->nm = 4;
     else if (m.nmax <= 16) h->nm = 16;
@@ -917,6 +928,35 @@ int w2d_ctqmc_create(const char *params, int nbands, int nftau, const double *u_
         (st = upload(h.get(), legc, &d.legc)) || (st = upload(h.get(), trans, &d.trans)) ||
         (st = upload(h.get(), opoff32, &d.opoff32)) || (h->nm > 16 && (st = build_ops16(h.get()))))
         return st;
+    // user symmetry moves (Trace.F90:471-478): SymMove01.. are 1-based
+    // flavour permutations; their inverses follow in the table
+    {
+        const long long nsym = P.has("NSymMove") ? P.integer("NSymMove") : 0;
+        std::vector<int> tab;
+        if (nsym > 0) {
+            tab.assign((size_t)2 * nsym * nf, 0);
+            for (long long i = 0; i < nsym; ++i) {
+                char key[16];
+                std::snprintf(key, sizeof(key), "SymMove%02lld", i + 1);
+                std::istringstream in(P.str(key));
+                std::vector<int> perm;
+                for (int v; in >> v;) perm.push_back(v);
+                std::vector<int> seen(nf + 1, 0);
+                bool ok = (int)perm.size() == nf;
+                for (int v : perm) ok = ok && v >= 1 && v <= nf && !seen[v]++;
+                if (!ok) {
+                    std::fprintf(stderr, "w2d_ctqmc: %s is not a permutation of the %d flavours\n", key, nf);
+                    return W2D_ERR_ARG;
+                }
+                for (int j = 0; j < nf; ++j) {
+                    tab[(size_t)i * nf + j] = perm[j];
+                    tab[(size_t)(nsym + i) * nf + perm[j] - 1] = j + 1;
+                }
+            }
+            if ((st = upload(h.get(), tab, &d.symmoves))) return st;
+        }
+        d.nsym = (int)std::max<long long>(0, nsym);
+    }
     d.p_shift = P.real("PercentageTauShiftMove");
     d.p_glob = P.real("PercentageGlobalMove");
     d.p_outer = 0.0;
@@ -1369,7 +1409,7 @@ int w2d_ctqmc_autotune(w2d_ctqmc *h, int64_t moves, int *nm, double *rates)
     double best_rate = -1.0;
     for (int i = 0; i < 4; ++i) {
         if (rates) rates[i] = 0.0;
-        if (cand[i] < h->nm_min || (cand[i] > 16 && h->dm.ntheta > 0)) continue;
+        if (cand[i] < h->nm_min || (cand[i] > 16 && (h->dm.ntheta > 0 || h->nm_min <= 16))) continue;
         int st = switch_engine(h, cand[i]);
         if (st == W2D_ERR_CAPACITY) continue;   // this engine's LDS does not fit
         if (st) return st;

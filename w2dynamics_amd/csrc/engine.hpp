@@ -8,6 +8,7 @@ namespace w2d {
 
 constexpr int WAVE = 64;       // one wavefront drives one Markov chain
 constexpr int MAXF = 14;       // flavours (2*nbands, nbands <= 7)
+constexpr int MAXSST = 1024;   // superstates (quantum-number blocks) a model may have
 
 // read-only model and parameters, passed by value to the kernels
 struct DevModel {
@@ -39,6 +40,10 @@ struct DevModel {
     // adds the score-function sums to every measurement
     int ntheta, score;
     const double *dblk, *dftau;
+    // user symmetry moves (NSymMove, SymMove01..): [2 nsym][nf] 1-based
+    // flavour permutations, each one's inverse in row nsym + i
+    int nsym;
+    const int *symmoves;
 };
 
 // per-chain scalar state (kept contiguously for all chains, after the chain blocks)

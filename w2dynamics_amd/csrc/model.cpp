@@ -193,14 +193,83 @@ static bool hamiltonian_blocks(int nb, const double *umat, const double *muimp, 
     return ok;
 }
 
+// analyze_hamiltonian (Operator.F90:303-442): within every block of s2s (its
+// states blk, H in the occupation basis h), the connected components of
+// |H_ij| > thr, numbered in order of discovery (breadth-first from the block's
+// lowest unassigned state); then, while some operator c or c+ maps two states
+// of one block into two different blocks, those two blocks are merged (the
+// larger label into the smaller, labels above it shifted down), rescanning
+// from the first operator after every merge.  Returns the number of blocks.
+static int refine_blocks(int nb, const std::vector<std::vector<int>> &blk,
+                         const std::vector<std::vector<double>> &h, double thr, std::vector<int> &s2s)
+{
+    const int ns = 1 << (2 * nb);
+    s2s.assign(ns, -1);
+    int nsub = -1;
+    for (size_t sst = 0; sst < blk.size(); ++sst) {
+        const std::vector<int> &st = blk[sst];
+        const int n = (int)st.size();
+        const std::vector<double> &H = h[sst];
+        std::vector<std::vector<int>> nets{{0}};
+        s2s[st[0]] = ++nsub;
+        for (size_t k = 0; k < nets.size(); ++k) {
+            for (size_t q = 0; q < nets[k].size(); ++q) {
+                const int node = nets[k][q];
+                for (int edge = 0; edge < n; ++edge) {
+                    const double v = H[node + (size_t)edge * n];
+                    if (v == 0.0 || !(std::fabs(v) > thr)) continue;
+                    if (std::find(nets[k].begin(), nets[k].end(), edge) == nets[k].end()) {
+                        nets[k].push_back(edge);
+                        s2s[st[edge]] = nsub;
+                    }
+                }
+            }
+            for (int node = 0; node < n; ++node)
+                if (s2s[st[node]] == -1) {
+                    nets.push_back({node});
+                    s2s[st[node]] = ++nsub;
+                    break;
+                }
+        }
+    }
+    std::vector<int> cs(ns);
+    for (bool change = true; change;) {
+        change = false;
+        for (int ib = 0; ib < nb && !change; ++ib)
+            for (int sp = 0; sp < 2 && !change; ++sp)
+                for (int ca = 1; ca <= 2 && !change; ++ca) {
+                    const int bit = ib + nb * sp;
+                    for (int i = 0; i < ns; ++i) {
+                        const int occ = (i >> bit) & 1;
+                        cs[i] = ((ca == 1 && !occ) || (ca == 2 && occ)) ? s2s[i ^ (1 << bit)] : -1;
+                    }
+                    for (int i1 = 0; i1 < ns && !change; ++i1) {
+                        if (cs[i1] == -1) continue;
+                        for (int i2 = i1 + 1; i2 < ns; ++i2) {
+                            if (s2s[i2] != s2s[i1] || cs[i2] == -1 || cs[i2] == cs[i1]) continue;
+                            const int big = std::max(cs[i1], cs[i2]), small = std::min(cs[i1], cs[i2]);
+                            for (int i = 0; i < ns; ++i) {
+                                if (s2s[i] == big) s2s[i] = small;
+                                if (s2s[i] > big) --s2s[i];
+                            }
+                            change = true;
+                            break;
+                        }
+                    }
+                }
+    }
+    return *std::max_element(s2s.begin(), s2s.end()) + 1;
+}
+
 std::string build_local_model(int nb, const double *umat, const double *muimp,
-                              const std::string &qns, LocalModel &m)
+                              const std::string &qns, LocalModel &m, double epsblock)
 {
     if (nb < 1 || nb > 7) return "nbands must be 1..7";
     const bool q_nt = qns.find("Nt") != std::string::npos;
     const bool q_sz = qns.find("Szt") != std::string::npos;
     const bool q_qz = qns.find("Qzt") != std::string::npos;
-    for (const char *bad : {"Azt", "Lzt", "Jzt", "All"})
+    const bool q_all = qns.find("All") != std::string::npos;
+    for (const char *bad : {"Azt", "Lzt", "Jzt"})
         if (qns.find(bad) != std::string::npos)
             return std::string("unsupported quantum number ") + bad;
     const int ns = 1 << (2 * nb);
@@ -232,6 +301,32 @@ std::string build_local_model(int nb, const double *umat, const double *muimp,
             s2s[b] = nsst;
         }
         ++nsst;
+    }
+    if (q_all) {
+        // "All" (CTQMC.F90:7971-7989): H on the blocks above, its connectivity
+        // refines them; threshold EPSBLOCK relative to the largest input element
+        std::vector<std::vector<int>> blk0(nsst);
+        std::vector<int> iib0(ns), sizes0(nsst);
+        for (int st = 0; st < ns; ++st) {
+            iib0[st] = (int)blk0[s2s[st]].size();
+            blk0[s2s[st]].push_back(st);
+        }
+        for (int s = 0; s < nsst; ++s) sizes0[s] = (int)blk0[s].size();
+        std::vector<std::vector<double>> h0;
+        hamiltonian_blocks(nb, umat, muimp, s2s, blk0, iib0, sizes0, false, h0);
+        for (int s = 0; s < nsst; ++s) {   // force_hermitian
+            const int n = sizes0[s];
+            std::vector<double> &a = h0[s];
+            for (int i = 0; i < n; ++i)
+                for (int j = i + 1; j < n; ++j) {
+                    const double x = 0.5 * (a[i + (size_t)j * n] + a[j + (size_t)i * n]);
+                    a[i + (size_t)j * n] = a[j + (size_t)i * n] = x;
+                }
+        }
+        double big = 0.0;
+        for (int i = 0; i < 4 * nb * nb; ++i) big = std::max(big, std::fabs(muimp[i]));
+        for (long i = 0; i < 16L * nb * nb * nb * nb; ++i) big = std::max(big, std::fabs(umat[i]));
+        nsst = refine_blocks(nb, blk0, h0, epsblock * big, s2s);
     }
     m.nsst = nsst;
     m.state_sst = s2s;

@@ -51,8 +51,10 @@ struct LocalModel {
 };
 
 // Builds the model; returns an empty string on success, else an error message.
+// Quantum numbers: any of Nt, Szt, Qzt, plus All (the blocks refined by the
+// connectivity of H with threshold epsblock times the largest input element).
 std::string build_local_model(int nbands, const double *umatrix_2n4, const double *muimp_n2n2,
-                              const std::string &quantum_numbers, LocalModel &m);
+                              const std::string &quantum_numbers, LocalModel &m, double epsblock = 1e-13);
 
 // d H_loc / d theta in the eigenbasis for inputs linear in (U matrix, muimp):
 // V_s^T (H(dumat, dmuimp))_s V_s per block, column-major, concatenated in block

@@ -341,7 +341,8 @@ class Problem:
                 "ParaMag": 0, "QuantumNumbers": self.quantum_numbers, "Phonon": 0,
                 "Screening": 0, "Uw": 0}
         s += "".join("%s = %s #\n" % (k, prob[k]) for k in sorted(prob))
-        s += "NSymMove = 0 #\n"
+        if "NSymMove" not in q:
+            s += "NSymMove = 0 #\n"
         return s
 
     def umatrix_flat(self) -> np.ndarray:
