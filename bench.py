@@ -444,7 +444,7 @@ def main():
     ap.add_argument("--therm-s", type=float, default=240.0, help="thermalisation time budget")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graphs", action="store_true", help="plain stream launches instead of one HIP graph per pass")
-    ap.add_argument("--no-order", action="store_true", help="launch chains in index order (no longest-first order)")
+    ap.add_argument("--order", action="store_true", help="order the chains longest first before every launch")
     ap.add_argument("--engine", type=int, default=-1, help="trace engine (padded block size); 0: the model's smallest, "
                     "-1: autotune")
     ap.add_argument("--no-plain", action="store_true")
@@ -472,7 +472,7 @@ def main():
     print("batched run: %s, %d chains" % (cfg, nchains), file=sys.stderr, flush=True)
     try:
         b = run_gpu(cfg, nchains, capacity, args.warmup, args.steps, args.ncorr, seed, args.therm_s,
-                    launch=dict(graphs=not args.no_graphs, order_chains=not args.no_order), engine=args.engine)
+                    launch=dict(graphs=not args.no_graphs, order_chains=args.order), engine=args.engine)
     except BaseException:
         if cpu_run:
             cpu_run.cancel()
@@ -536,7 +536,7 @@ def main():
             "capacity": st["capacity"], "max_order_per_flavour": int(st["kfmax"].max()),
             "chains_failed": int((st["status"] != 0).sum()),
             "trace_engine": b["engine"][0], "autotune_moves_per_s": b["tuned"],
-            "launch": {"hip_graph": not args.no_graphs, "chains_longest_first": not args.no_order}},
+            "launch": {"hip_graph": not args.no_graphs, "chains_longest_first": args.order}},
         "gtau_inv_err2_per_s": inv_err2_s,
         "mean_sign": b["mean_sign"],
         "plain_single_chain": plain,

@@ -248,7 +248,9 @@ int w2d_ctqmc_trap_signals(void);
  * graphs != 0: the sweep launches of a warmup / measure pass are captured into
  * one HIP graph and launched together (default on); order_chains != 0: before
  * every sweep launch the chains are ordered by expansion order, longest first,
- * so that a launch's tail is short (default on); moves_per_launch: moves per
+ * so that a launch's tail is short (default off: runs of the trace engine for
+ * blocks above 16 states with it on ended in GPU memory faults);
+ * moves_per_launch: moves per
  * chain in one sweep launch (rounded to whole measurement intervals; 0 = the
  * default of about 4000).  None of them changes any chain's results. */
 int w2d_ctqmc_set_launch(w2d_ctqmc *h, int graphs, int order_chains, int64_t moves_per_launch);

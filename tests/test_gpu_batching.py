@@ -64,19 +64,30 @@ def _run(prob, tdm, **launch):
     return acc, cfg
 
 
-def test_graphs_and_chain_order_change_no_result():
-    """Many short launches (2 sweeps each) with and without graph capture and
-    chain ordering: identical configurations, accumulators to rounding."""
+# Chain ordering (order_chains) is off by default: every GPU memory fault seen
+# with this library came from runs with it on (trace engine for blocks above 16
+# states); its variants stay skipped until that is explained.
+ORDERING_OFF = "longest-first chain ordering is switched off: runs with it on ended in GPU memory faults"
+LAUNCH_OPTS = [
+    pytest.param(dict(graphs=True, order_chains=False), id="graphs"),
+    pytest.param(dict(graphs=False, order_chains=True), id="order", marks=pytest.mark.skip(reason=ORDERING_OFF)),
+    pytest.param(dict(graphs=True, order_chains=True), id="graphs_order",
+                 marks=pytest.mark.skip(reason=ORDERING_OFF)),
+]
+
+
+@pytest.mark.parametrize("opts", LAUNCH_OPTS)
+def test_graphs_and_chain_order_change_no_result(opts):
+    """Many short launches (2 sweeps each) with graph capture (and chain
+    ordering): identical configurations, accumulators to rounding."""
     prob = P.kanamori_2orb(beta=20.0, TaudiffMax=4.0, NCorr=50)
     ref, rcfg = _run(prob, 4.0, graphs=False, order_chains=False, moves_per_launch=100)
-    for opts in (dict(graphs=True, order_chains=False), dict(graphs=False, order_chains=True),
-                 dict(graphs=True, order_chains=True)):
-        print("launch options", opts, flush=True)
-        acc, cfg = _run(prob, 4.0, moves_per_launch=100, **opts)
-        for a, b in zip(cfg, rcfg):
-            assert a["n"] == b["n"] and np.array_equal(a["taus"], b["taus"]), opts
-        for k in ("gtau", "gleg", "dm", "histo", "scalars"):
-            assert np.allclose(acc[k], ref[k], rtol=1e-12, atol=1e-12), (opts, k)
+    print("launch options", opts, flush=True)
+    acc, cfg = _run(prob, 4.0, moves_per_launch=100, **opts)
+    for a, b in zip(cfg, rcfg):
+        assert a["n"] == b["n"] and np.array_equal(a["taus"], b["taus"]), opts
+    for k in ("gtau", "gleg", "dm", "histo", "scalars"):
+        assert np.allclose(acc[k], ref[k], rtol=1e-12, atol=1e-12), (opts, k)
 
 
 @pytest.mark.skip(reason=MFMA_SMALL_BLOCKS)

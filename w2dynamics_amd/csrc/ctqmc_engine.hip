@@ -51,7 +51,7 @@ struct w2d_ctqmc {
     // moves per chain and launch (0: whole measurement intervals of ~4000)
     int nm_min = 0;            // smallest trace engine (padded block size) the model allows
     int *order = nullptr;      // device: chain of each workgroup
-    bool order_chains = true, graphs = true;
+    bool order_chains = false, graphs = true;   // chain ordering: opt-in
     long long per_launch = 0;
     ~w2d_ctqmc();
 };

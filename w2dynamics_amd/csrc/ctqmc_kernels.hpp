@@ -3439,8 +3439,13 @@ __global__ void __launch_bounds__(WAVE) sweep_kernel(DevModel M, ChainLayout L, 
                                                      const int *order)
 {
     // order: the chain each workgroup runs (longest first, so that the
-    // launch's tail is short), or null for workgroup = chain
-    const int chain = order ? order[blockIdx.x] : (int)blockIdx.x;
+    // launch's tail is short), or null for workgroup = chain.  The order is
+    // rewritten by order_kernel before every launch: it is read with a
+    // device-coherent vector load, never through the scalar cache (which is
+    // not coherent with the vector stores of an earlier kernel)
+    const int chain = order ? __builtin_amdgcn_readfirstlane(__hip_atomic_load(
+                                  order + blockIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                            : (int)blockIdx.x;
     if (only >= 0 && chain != only) return;
     extern __shared__ __align__(16) char lds[];
     char *base = chains + (long)chain * L.stride;
@@ -3620,7 +3625,10 @@ __global__ void __launch_bounds__(256) group_sum_kernel(const char *chains, long
     const int c0 = (int)((long)g * nc / ng), c1 = (int)((long)(g + 1) * nc / ng);
     double acc = 0.0;
     for (int c = c0; c < c1; ++c)
-        if (hdrs[c].err == 0) acc += reinterpret_cast<const double *>(chains + (long)c * stride + acc_off)[e];
+        // the error flag is written by the sweep kernels: a device-coherent
+        // load (a uniform address would otherwise go through the scalar cache)
+        if (__hip_atomic_load(&hdrs[c].err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+            acc += reinterpret_cast<const double *>(chains + (long)c * stride + acc_off)[e];
     out[(long)g * nel + e] = acc;
 }
 

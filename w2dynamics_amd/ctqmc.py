@@ -397,7 +397,7 @@ class CtqmcGpu:
 
 
     # -- launch machinery ---------------------------------------------------------
-    def set_launch(self, graphs: bool = True, order_chains: bool = True, moves_per_launch: int = 0):
+    def set_launch(self, graphs: bool = True, order_chains: bool = False, moves_per_launch: int = 0):
         """Sweep launches of a pass captured into one HIP graph; chains ordered
         longest first before every launch; moves per chain and launch (0: ~4000)."""
         _check(self.lib.w2d_ctqmc_set_launch(self.h, int(bool(graphs)), int(bool(order_chains)),
