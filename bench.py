@@ -1,13 +1,15 @@
 """Benchmark: CT-HYB Metropolis moves/s and G(tau) 1/err^2/s on one MI355X,
 batched chains vs the plain single-chain path.
 
-Workload (default, c4): 5-orbital full-Coulomb impurity (F0=4, J=0.8, F4/F2=0.625),
-filling 6, beta=40, seeded Gaussian crystal field, semicircular baths of
-half-bandwidth 2, 1024 local states in (N,Sz) blocks up to 100 x 100 (the local
-trace runs on the FP64 matrix cores), 2048 batched Markov chains, NCorr=100,
-TaudiffMax=5.  A "step" is one sweep: NCorr Metropolis moves of every chain
-followed by the Z-space measurements (G(tau) bins + Legendre, beta/2 density
-matrix, sign, histograms).
+Workload (default, c2): 2-orbital Kanamori impurity (U=4, J=1), half filling,
+beta=50, semicircular baths of half-bandwidth 2, 16 local states in (N,Sz) blocks
+of up to 4 states, 1024 batched Markov chains, NCorr=100, TaudiffMax=5.  Other
+workloads: c1 (1-orbital Anderson), c3 (3-orbital Kanamori, beta=100, 4096
+chains, blocks up to 9 x 9), c4 (5-orbital full-Coulomb impurity, F0=4, J=0.8,
+filling 6, beta=40, 1024 states in blocks up to 100 x 100 on the FP64 matrix
+cores, 2048 chains), c5 (c3 with score-function gradients).  A "step" is one
+sweep: NCorr Metropolis moves of every chain followed by the Z-space
+measurements (G(tau) bins + Legendre, beta/2 density matrix, sign, histograms).
 
 Before the timed region the chains are thermalised until the mean expansion
 order over all chains stops growing (whatever --warmup says; --warmup is the
@@ -19,7 +21,7 @@ baseline is the reference Fortran core built from its sources (oracle/_ref),
 one process per host core, each started from a thermalised configuration
 (set_mc_config, as ReuseMCConfig does) under a fixed measurement budget.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c4|c3|c2|c1]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c1|c3|c4|c5]
                        [--chains N] [--no-cpu] [--no-plain] [--therm-s S]
 
 Prints one JSON line with a roofline object for the sweep kernel (FP64 MFMA for
@@ -437,7 +439,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=10, help="minimum thermalisation sweeps")
-    ap.add_argument("--config", default="c4", choices=sorted(WORKLOADS) + ["c5"])
+    ap.add_argument("--config", default="c2", choices=sorted(WORKLOADS) + ["c5"])
     ap.add_argument("--chains", type=int, default=0)
     ap.add_argument("--ncorr", type=int, default=100)
     ap.add_argument("--seed", type=int, default=43890)

@@ -957,6 +957,12 @@ int w2d_ctqmc_create(const char *params, int nbands, int nftau, const double *u_
         }
         d.nsym = (int)std::max<long long>(0, nsym);
     }
+    // the MFMA engine (blocks above 16 states) keeps the global-move code of
+    // its validated builds: no user symmetry moves
+    if (h->nm > 16 && d.nsym > 0) {
+        std::fprintf(stderr, "w2d_ctqmc: NSymMove is not supported with blocks above 16 states\n");
+        return W2D_ERR_ARG;
+    }
     d.p_shift = P.real("PercentageTauShiftMove");
     d.p_glob = P.real("PercentageGlobalMove");
     d.p_outer = 0.0;

@@ -2315,11 +2315,14 @@ DEV int step_glob(Ctx &c)
     } else if (r < 0.6) {
         gu[0] = 1;
         for (int i = 1; i <= nf; ++i) gu[i] = i;
-    } else if (r < 0.8 && M.nsym > 0) {
+    } else if (NM <= 16 && r < 0.8 && M.nsym > 0) {
         // gen_SymUpdate (Trace.F90:1166-1182): a user permutation or its inverse
-        const int isym = randint(c.rng, 1, 2 * M.nsym);
-        gu[0] = 0;
-        for (int i = 1; i <= nf; ++i) gu[i] = M.symmoves[(isym - 1) * nf + (i - 1)];
+        // (engines up to 16 states; the MFMA engine refuses NSymMove)
+        if constexpr (NM <= 16) {
+            const int isym = randint(c.rng, 1, 2 * M.nsym);
+            gu[0] = 0;
+            for (int i = 1; i <= nf; ++i) gu[i] = M.symmoves[(isym - 1) * nf + (i - 1)];
+        }
     } else {
         for (int i = 1; i <= nf; ++i) gu[i] = i;
         for (int j = 1; j <= nf; ++j) {   // rand_perm (Trace.F90:1065-1080)
@@ -2334,7 +2337,7 @@ DEV int step_glob(Ctx &c)
     igu[0] = gu[0];
     const int old_outer = c.h.outer;
     double sst_factor;
-    unsigned long long okm[MAXSST / WAVE];
+    unsigned long long okm[MAXSST / WAVE];   // valid outer superstates, one bit each
     if (spinflip && M.has_szt) {
         // perform_spinflip_check_qn (Trace.F90:1306-1339)
         const int st = M.first_state[c.h.outer];

@@ -8,7 +8,7 @@ namespace w2d {
 
 constexpr int WAVE = 64;       // one wavefront drives one Markov chain
 constexpr int MAXF = 14;       // flavours (2*nbands, nbands <= 7)
-constexpr int MAXSST = 1024;   // superstates (quantum-number blocks) a model may have
+constexpr int MAXSST = 256;    // superstates (quantum-number blocks) a model may have
 
 // read-only model and parameters, passed by value to the kernels
 struct DevModel {
