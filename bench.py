@@ -7,9 +7,11 @@ of up to 4 states, 1024 batched Markov chains, NCorr=100, TaudiffMax=5.  Other
 workloads: c1 (1-orbital Anderson), c3 (3-orbital Kanamori, beta=100, 4096
 chains, blocks up to 9 x 9), c4 (5-orbital full-Coulomb impurity, F0=4, J=0.8,
 filling 6, beta=40, 1024 states in blocks up to 100 x 100 on the FP64 matrix
-cores, 2048 chains), c5 (c3 with score-function gradients).  A "step" is one
-sweep: NCorr Metropolis moves of every chain followed by the Z-space
-measurements (G(tau) bins + Legendre, beta/2 density matrix, sign, histograms).
+cores, 2048 chains), c5 (c3 with score-function gradients).  A "step" is 40
+sweeps of every chain for c1/c2 (one launch of the persistent sweep kernel), 4
+for c3 and 1 for c4 and c5, a sweep being NCorr Metropolis moves followed by the
+Z-space measurements (G(tau) bins + Legendre, beta/2 density matrix, sign,
+histograms).
 
 Before the timed region the chains are thermalised until the mean expansion
 order over all chains stops growing (whatever --warmup says; --warmup is the
@@ -54,6 +56,10 @@ WORKLOADS = {
     "c3": ("kanamori_3orb", dict(beta=100.0), 8.0, 4096, 384, 300),
     "c4": ("coulomb_5orb", dict(beta=40.0), 5.0, 2048, 64, 12),
 }
+# sweeps (NCorr moves + one measurement per chain) in one step: one launch of
+# the sweep kernel (40 sweeps = 4000 moves per chain) for the small-block
+# workloads, fewer where a sweep already takes tens of milliseconds
+SWEEPS_PER_STEP = {"c1": 40, "c2": 40, "c3": 4, "c4": 1}
 
 
 def make_problem(cfg, ncorr, **extra):
@@ -473,7 +479,8 @@ def main():
     cpu_run = None if args.no_cpu else CpuBaseline(cfg, args.ncorr, seed)
     print("batched run: %s, %d chains" % (cfg, nchains), file=sys.stderr, flush=True)
     try:
-        b = run_gpu(cfg, nchains, capacity, args.warmup, args.steps, args.ncorr, seed, args.therm_s,
+        b = run_gpu(cfg, nchains, capacity, args.warmup, args.steps * SWEEPS_PER_STEP[cfg], args.ncorr, seed,
+                    args.therm_s,
                     launch=dict(graphs=not args.no_graphs, order_chains=args.order), engine=args.engine)
     except BaseException:
         if cpu_run:
@@ -532,7 +539,7 @@ def main():
         "data": "synthetic (semicircular-bath impurity model, seeded crystal field)",
         "config": {"workload": "%s: %s, beta=%g, %d batched chains, NCorr=%d, TaudiffMax=%g" % (
             cfg, b["label"], b["beta"], nchains, args.ncorr, b["tdm"]),
-            "chains": nchains, "ncorr": args.ncorr,
+            "chains": nchains, "ncorr": args.ncorr, "sweeps_per_step": SWEEPS_PER_STEP[cfg],
             "mean_order_ops_before": b["order_before"], "mean_order_ops_after": b["order_after"],
             "thermalisation": [[s, round(o, 2), round(t, 1)] for s, o, t in b["traj"]],
             "capacity": st["capacity"], "max_order_per_flavour": int(st["kfmax"].max()),
