@@ -86,6 +86,18 @@ module MCTQMC
    integer(c_int), allocatable, private, save :: model_sizes(:), model_offs(:)
    real(KINDR), allocatable, private, save :: model_vec(:)
 
+   ! internal helpers and the C-ABI bindings are not part of the module's
+   ! interface (an f2py wrapper of the module sees the reference's names only)
+   private :: failed, param_value, load_model, collect_accpair, sort_ascending, collect_results, &
+              collect_histograms, apply_ops, reduced_density_matrices, collect_steps_and_blocks, &
+              reset_results
+   private :: w2d_ctqmc_aborted, w2d_ctqmc_calibrate, w2d_ctqmc_create, w2d_ctqmc_destroy, &
+              w2d_ctqmc_dims, w2d_ctqmc_get_accumulators, w2d_ctqmc_get_config, &
+              w2d_ctqmc_get_group_histograms, w2d_ctqmc_get_group_sums, w2d_ctqmc_get_gtau_blocks, &
+              w2d_ctqmc_get_model, w2d_ctqmc_get_pair_stats, w2d_ctqmc_get_steps, &
+              w2d_ctqmc_init_counters, w2d_ctqmc_measure, w2d_ctqmc_set_config, &
+              w2d_ctqmc_set_taudiffmax, w2d_ctqmc_trap_signals, w2d_ctqmc_warmup
+
    interface
       integer(c_int) function w2d_ctqmc_create(params, nbands, nftau, u, ftau, muimp, nchains, &
                                                seed0, capacity, device, out) &
