@@ -89,6 +89,32 @@ if "rho2" in result:
     result["rho1"] = result["rho1"].reshape((norb, 2) * 2)
 for k, v in result.items():
     out["rs_" + k] = np.asarray(v)
+if len(sys.argv) > 3 and sys.argv[3] == "reuse":
+    # ReuseMCConfig (cthyb_solver.py:24-86, 328-332): the last configuration
+    # goes into the next solve right after init_solver
+    n, osst, ostate = m.get_mc_config_scalars()
+    cfg = m.get_mc_config(n) if n > 0 else None
+    m.dest_ctqmc()
+    m.init_solver(np.asfortranarray(np.asarray(z["umatrix"]).reshape(n2, n2, n2, n2)), ftau,
+                  np.asfortranarray(z["muimp"], dtype=np.float64), np.zeros_like(ftau), 2)
+    assert int(m.aborted) == 0, "init_solver 2"
+    if cfg is not None:
+        m.set_mc_config(n, *cfg[:5], cfg[5], cfg[6])
+    else:
+        m.set_empty_mc_config(osst, ostate)
+    assert int(m.aborted) == 0, "set_mc_config"
+    n2_, osst2, _ = m.get_mc_config_scalars()
+    back = m.get_mc_config(n2_) if n2_ > 0 else None
+    out["reuse_n"] = np.array([n, n2_, osst, osst2])
+    if cfg is not None:
+        for i, k in enumerate(("taus", "orbs", "spins", "cas", "hashybs")):
+            out["reuse_set_" + k] = np.asarray(cfg[i])
+            out["reuse_back_" + k] = np.asarray(back[i])
+    m.set_taudiffmax(float(z["taudiff_max_param"]))
+    m.ctqmc_warmup(100)
+    m.ctqmc_measure(1, 1)
+    assert int(m.aborted) == 0, "second solve"
+    out["reuse_gtau"] = np.asarray(m.gtau)
 m.dest_ctqmc()
 m.dest_paras()
 np.savez(sys.argv[2], **out)
@@ -122,3 +148,28 @@ def test_f2py_module_reproduces_reference_run(tmp_path, name):
     assert got["rs_hist-sst"].shape == (int(z["model_nsst"]),)
     for k in RESULTSET:
         assert "rs_" + k in got.files, k
+
+
+@pytest.mark.gpu
+def test_f2py_module_reuses_the_mc_configuration(tmp_path):
+    """ReuseMCConfig through the extension: the configuration read after one
+    solve (get_mc_config_scalars, get_mc_config) is set right after the next
+    init_solver (set_mc_config(N, ...)), reads back unchanged, and the second
+    solve runs from it."""
+    import os
+    import subprocess
+    import sys
+    from conftest import ROOT
+    script = DRIVE % dict(root=ROOT, tests=os.path.join(ROOT, "tests"), resultset=RESULTSET, density=DENSITY)
+    out = str(tmp_path / "out.npz")
+    print("f2py module reuse", flush=True)
+    r = subprocess.run([sys.executable, "-c", script, "c2_run", out, "reuse"], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    got = np.load(out)
+    n, n_back, osst, osst_back = got["reuse_n"]
+    assert n > 0 and n_back == n and osst_back == osst
+    for k in ("taus", "orbs", "spins", "cas", "hashybs"):
+        assert np.array_equal(got["reuse_set_" + k], got["reuse_back_" + k]), k
+    assert np.all(got["reuse_set_hashybs"] == -1)
+    assert np.all(np.isfinite(got["reuse_gtau"]))

@@ -719,6 +719,7 @@ contains
       integer, intent(in) :: N
       real(KINDR), dimension(N), intent(out) :: taus
       integer, dimension(N), intent(out) :: orbs, spins, cas, hashybs
+      !f2py depend(N) taus, orbs, spins, cas, hashybs
       integer, intent(out) :: outer_sst, outer_state
       integer(c_int) :: nn, osst
       integer(c_int) :: o(max(N, 1)), sp(max(N, 1)), ca(max(N, 1))
@@ -744,6 +745,8 @@ contains
       integer, intent(in) :: N
       real(KINDR), dimension(N), intent(in) :: taus
       integer, dimension(N), intent(in) :: orbs, spins, cas, hashybs
+      ! N first and required, as cthyb_solver.py passes it (CtHybConfig.set_to_ctqmc)
+      !f2py depend(N) taus, orbs, spins, cas, hashybs
       integer, intent(in) :: outer_sst, outer_state
       integer(c_int) :: o(max(N, 1)), sp(max(N, 1)), ca(max(N, 1))
       real(KINDR) :: t(max(N, 1)), w(5)
