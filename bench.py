@@ -456,6 +456,7 @@ def main():
     ap.add_argument("--engine", type=int, default=-1, help="trace engine (padded block size); 0: the model's smallest, "
                     "-1: autotune")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--c4-anyway", action="store_true", help="run the 5-orbital workload although it is switched off")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run 5 device copies of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -471,6 +472,13 @@ def main():
     if cfg == "c5":
         run_c5(args)
         return
+    if cfg == "c4" and not args.c4_anyway:
+        # the 5-orbital thermalisation at 2048 chains ended in GPU memory faults
+        # (twice at the same point of the seeded run) and its cause is not
+        # shown: it is not run unless asked for explicitly
+        print("bench.py: --config c4 is switched off (its thermalisation faulted the GPU; cause not "
+              "shown); pass --c4-anyway to run it", file=sys.stderr)
+        sys.exit(2)
     nchains = args.chains or WORKLOADS[cfg][3]
     capacity = WORKLOADS[cfg][4]
     seed = args.seed
