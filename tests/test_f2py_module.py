@@ -173,3 +173,39 @@ def test_f2py_module_reuses_the_mc_configuration(tmp_path):
         assert np.array_equal(got["reuse_set_" + k], got["reuse_back_" + k]), k
     assert np.all(got["reuse_set_hashybs"] == -1)
     assert np.all(np.isfinite(got["reuse_gtau"]))
+
+
+def test_f2py_module_reports_library_errors(tmp_path):
+    """An unsupported parameter (state sampling): init_solver reports the
+    library's status and sets aborted = 2 instead of stopping the process, as
+    the Fortran host does; the module stays usable (dest_ctqmc, dest_paras)."""
+    import os
+    import subprocess
+    import sys
+    from conftest import ROOT
+    script = r"""
+import sys
+import numpy as np
+sys.path.insert(0, %(root)r)
+sys.path.insert(0, %(tests)r)
+from conftest import load_golden
+from w2dynamics_amd import CTQMC
+m = CTQMC.mctqmc
+z = load_golden("c1_run")
+lines = [("statesampling = 1" if ln.split("=")[0].strip() == "statesampling" else ln)
+         for ln in str(z["param_string"]).split("\n")]
+assert "statesampling = 1" in lines
+m.init_paras("\n".join(lines))
+m.init_rnd(1)
+nb = int(z["nbands"]); n2 = 2 * nb
+ftau = np.asfortranarray(z["ftau"], dtype=np.float64)
+m.init_solver(np.asfortranarray(np.asarray(z["umatrix"]).reshape(n2, n2, n2, n2)), ftau,
+              np.asfortranarray(z["muimp"], dtype=np.float64), np.zeros_like(ftau), 1)
+print("aborted", int(m.aborted))
+m.dest_ctqmc()
+m.dest_paras()
+""" % dict(root=ROOT, tests=os.path.join(ROOT, "tests"))
+    r = subprocess.run([sys.executable, "-c", script], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "aborted 2" in r.stdout
+    assert "w2d_ctqmc_create returned status" in r.stderr
