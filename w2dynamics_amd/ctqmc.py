@@ -290,6 +290,32 @@ class CtqmcGpu:
                                              _p(cas), int(outer_sst), _p(w)), "set_config")
         return dict(trace_log=w[0], trace_sign=w[1], det_log=w[2], det_sign=w[3], cfgsign=w[4])
 
+    # ReuseMCConfig for a batch (cthyb_solver.py:24-86, 328-332 keep one
+    # configuration per rank): every chain's configuration, written to and read
+    # from one .npz, so that the next solve starts each chain where it stopped
+    def save_configs(self, path: str):
+        cfgs = [self.get_config(c) for c in range(self.nchains)]
+        n = np.array([c["n"] for c in cfgs], np.int64)
+        cat = (lambda k, dt: np.concatenate([np.asarray(c[k], dt) for c in cfgs]) if n.sum() else
+               np.zeros(0, dt))
+        np.savez(path, n=n, outer_sst=np.array([c["outer_sst"] for c in cfgs], np.int32),
+                 taus=cat("taus", np.float64), orbs=cat("orbs", np.int32), spins=cat("spins", np.int32),
+                 cas=cat("cas", np.int32))
+
+    def load_configs(self, path: str) -> list:
+        """Set every chain to the configuration saved for it (chains beyond the
+        saved ones keep theirs); returns the weights set_config computed."""
+        with np.load(path) as z:
+            n, outer = z["n"], z["outer_sst"]
+            off = np.concatenate([[0], np.cumsum(n)])
+            arrs = {k: z[k] for k in ("taus", "orbs", "spins", "cas")}
+        out = []
+        for c in range(min(self.nchains, len(n))):
+            a, b = int(off[c]), int(off[c + 1])
+            out.append(self.set_config(c, arrs["taus"][a:b], arrs["orbs"][a:b], arrs["spins"][a:b],
+                                       arrs["cas"][a:b], int(outer[c])))
+        return out
+
     def get_minv(self, chain: int, f: int) -> np.ndarray:
         k = C.c_int()
         _check(self.lib.w2d_ctqmc_get_minv(self.h, chain, f, C.byref(k), None), "get_minv")
