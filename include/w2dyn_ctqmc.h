@@ -173,6 +173,21 @@ int w2d_ctqmc_get_group_sums_noalloc(w2d_ctqmc *h, int ngroups, double *gtau, do
  * handle was created with AccumGtau = 2. */
 int w2d_ctqmc_get_gtau_blocks(w2d_ctqmc *h, int chain, int *nlevels, double *blocks);
 
+/* MeasGiw (measure_giw_nfft, CTQMC.F90:1015-1059, with the MeasGiwFlushLookup
+ * path's sign, CTQMC.F90:2403-2417): raw sums over the measurements of
+ *   cfgsign * sum_{a,c} M[c,a] exp(i w_n (tau_a - tau_c)),  w_n = (2n+1) pi / beta,
+ * for n = 0..Niw-1 (G(i w_{-n-1}) is the complex conjugate), evaluated as the
+ * direct sum (the NFFT fallback of Nfft_z.F90:64-90) on the GPU after every
+ * measurement.  The normalised Giw of CTQMC.F90:8816 is sum / (count x mean
+ * sign x beta) with the scalars of the same chains.  ngroups = 0: per chain,
+ * out[nchains][2N][Niw][2] (re, im); ngroups > 0: sums over the contiguous
+ * groups of chains of w2d_ctqmc_get_group_sums (chains stopped on an error left
+ * out), out[ngroups][2N][Niw][2].  *niw = Niw (0 when the handle was created
+ * without MeasGiw); out = NULL queries it.  W2D_ERR_STATE with MeasGiw off or
+ * before a measure.  With MeasGiw on, measure launches end on every
+ * measurement (one launch per NCorr moves). */
+int w2d_ctqmc_get_giw(w2d_ctqmc *h, int ngroups, int *niw, double *out);
+
 /* Eigenbasis of the local Hamiltonian built by create: per-superstate sizes and
  * offsets, eigenvalues (EigenstatesEnergies), occupation-basis state of each
  * index (States), and the block eigenvectors (column-major, concatenated);

@@ -155,6 +155,9 @@ typedef struct {
     /* accumulators */
     double *gtau_buf, *gleg, *dm, *histo, *outer_hist, *outer_state_hist;
     double *sign_hist_sst, *sign_hist_states, *tc_sst, *tc_states;
+    /* MeasGiw: raw G(i w_n) sums [nf][niw][2] (re, im) for n = 0..niw-1 (niw = 0: off) */
+    int niw;
+    double *giw;
     double meas_sign;
     long cnt_meas;
     /* accepted-pair statistics (CTQMC.F90:66-70): AccPair [nf][ACC_CAP],
@@ -1179,6 +1182,21 @@ static void meas_gtau(ochain_t *ch)
             if (FL(*c) != f || c->ca != 1) continue;
             cpos[f]++;
             double M = ch->minv[f][(cpos[f] - 1) + (long)(apos[f] - 1) * k];
+            /* MeasGiw with NLookup_nfft = 0: measure_giw_nfft (CTQMC.F90:1015-1059)
+             * sums M(c,a) exp(i w_n (tau_a - tau_c)) over every pair of the
+             * flavour, w_n = (2n+1) pi / beta, with the direct sum of the NFFT
+             * fallback (Nfft_z.F90:64-90); the sign convention is that of the
+             * lookup path (CTQMC.F90:2403-2417 with :8816): cfgsign M, so that
+             * G(i w) is the Fourier transform of the G(tau) and G_l estimators */
+            if (ch->niw > 0) {
+                const double mw = M * ch->cfgsign, d = a->tau - c->tau;
+                double *gw = ch->giw + (long)f * ch->niw * 2;
+                for (int n = 0; n < ch->niw; ++n) {
+                    const double ph = 3.14159265358979323846 * (2.0 * n + 1.0) * d / ch->beta;
+                    gw[2 * n] += mw * cos(ph);
+                    gw[2 * n + 1] += mw * sin(ph);
+                }
+            }
             int idt;
             double x;
             if (a->tau - c->tau > 0.0) {
@@ -1667,6 +1685,8 @@ void *oracle_create(int nb, int nsst, int nmax, const int *sizes, const int *off
     /* iparams[8..10] Gtau_mean_step, Gtau_mid_step, sign_step (arrays of Nmeas) */
     ch->step_mean = iparams[8] != 0, ch->step_mid = iparams[9] != 0, ch->step_sign = iparams[10] != 0;
     ch->nmstep = (ch->step_mean || ch->step_mid || ch->step_sign) ? (int)iparams[7] : 0;
+    /* iparams[11] Niw when MeasGiw is set, else 0 */
+    ch->niw = (int)iparams[11];
     ch->cap = 1024;
     ch->op = malloc(sizeof(oop_t) * (size_t)ch->cap);
     ch->ket = malloc(sizeof(double) * (size_t)nmax);
@@ -1681,6 +1701,7 @@ void *oracle_create(int nb, int nsst, int nmax, const int *sizes, const int *off
     for (int f = 0; f < nf; ++f) ch->minv[f] = malloc(sizeof(double)), ch->k[f] = 0;
     ch->gtau_buf = calloc((size_t)nf * ch->ntau, sizeof(double));
     ch->gleg = calloc((size_t)nf * ch->nleg, sizeof(double));
+    ch->giw = calloc((size_t)nf * (ch->niw > 0 ? ch->niw : 1) * 2, sizeof(double));
     ch->gmeas = calloc((size_t)nf * ch->ntau, sizeof(double));
     ch->accpair = calloc((size_t)nf * ACC_CAP, sizeof(double));
     ch->aptau = calloc(APT_CAP, sizeof(double));
@@ -1735,6 +1756,7 @@ void oracle_destroy(void *h)
     for (int f = 0; f < NF(ch); ++f) free(ch->minv[f]);
     free(ch->op), free(ch->ket), free(ch->tmp), free(ch->kets_b2), free(ch->bras_b2);
     free(ch->klog_b2), free(ch->blog_b2), free(ch->parttrace), free(ch->tpart);
+    free(ch->giw);
     free(ch->gtau_buf), free(ch->gleg), free(ch->dm), free(ch->histo), free(ch->outer_hist);
     free(ch->gmeas), free(ch->acc_spill), free(ch->acc_mean), free(ch->acc_sq);
     free(ch->accpair), free(ch->aptau), free(ch->mstep), free(ch->dstep), free(ch->sstep);
@@ -1889,6 +1911,14 @@ void oracle_measure(void *h, long nmeas, long ncorr)
         meas_gtau(ch);
         if (ch->score) meas_score(ch);
     }
+}
+
+/* raw G(i w_n) sums [nf][niw][2]; returns niw (0: MeasGiw off) */
+int oracle_giw(void *h, double *out)
+{
+    ochain_t *ch = h;
+    if (out && ch->niw > 0) memcpy(out, ch->giw, sizeof(double) * (size_t)NF(ch) * ch->niw * 2);
+    return ch->niw;
 }
 
 /* binning blocks of G(tau) (AccumGtau = 2): [nlev][nf*ntau]; returns nlev */

@@ -54,6 +54,8 @@ def lib():
         _lib.oracle_get_minv.restype = C.c_int
         _lib.oracle_set_config.argtypes = [P, C.c_int, P, P, P, P, C.c_int, P]
         _lib.oracle_rng_stream.argtypes = [C.c_uint, C.c_long, P, P, P, P]
+        _lib.oracle_giw.argtypes = [P, P]
+        _lib.oracle_giw.restype = C.c_int
         _lib.oracle_gtau_blocks.argtypes = [P, P]
         _lib.oracle_gtau_blocks.restype = C.c_int
         _lib.oracle_autocorr.argtypes = [C.c_long, C.c_int, C.c_long, P, P, P]
@@ -134,7 +136,9 @@ class OracleChain:
                                           int(qmc["Nmeas"]),
                                           int(qmc.get("Gtau_mean_step", 0)),
                                           int(qmc.get("Gtau_mid_step", 0)),
-                                          int(qmc.get("sign_step", 0))], dtype=np.int64)
+                                          int(qmc.get("sign_step", 0)),
+                                          int(qmc.get("Niw", 2000)) if int(qmc.get("MeasGiw", 0)) else 0],
+                                         dtype=np.int64)
         for key in ("opoff", "evoff"):
             self._keep[key] = np.ascontiguousarray(self._keep[key], dtype=np.int64)
         k = self._keep
@@ -287,6 +291,15 @@ class OracleChain:
         lib().oracle_gtau_blocks(self.h, _p(out))
         return out.reshape(nlev, 2 * self.nb, self.ntau)
 
+    def giw_raw(self) -> np.ndarray:
+        """MeasGiw: raw sums of cfgsign M(c,a) exp(i w_n (tau_a - tau_c)) over the
+        measurements, [flavour][n] for n = 0..Niw-1 (complex)."""
+        niw = lib().oracle_giw(self.h, None)
+        out = np.zeros(2 * self.nb * max(niw, 1) * 2)
+        lib().oracle_giw(self.h, _p(out))
+        out = out[:2 * self.nb * niw * 2].reshape(2 * self.nb, niw, 2)
+        return out[..., 0] + 1j * out[..., 1]
+
     def raw(self) -> dict:
         nf = 2 * self.nb
         nst = 4 ** self.nb
@@ -341,5 +354,12 @@ class OracleChain:
                         occ[b1, s1, b2, s2] = np.sum(diag * n1 * n2)
         acc = np.array([sc[2] / sc[3] if sc[3] else np.nan, sc[4] / sc[5] if sc[5] else np.nan,
                         sc[6] / sc[7] if sc[7] else np.nan, sc[8] / sc[9] if sc[9] else np.nan])
-        return dict(mean_sign=mean_sign, gtau=gtau, gleg=gleg, occ=occ, densitymatrix=dm_occ,
-                    histo=fl2(r["histo"]) / cnt, acc=acc, outer_sst_histo=r["outer_hist"] / cnt)
+        out = dict(mean_sign=mean_sign, gtau=gtau, gleg=gleg, occ=occ, densitymatrix=dm_occ,
+                   histo=fl2(r["histo"]) / cnt, acc=acc, outer_sst_histo=r["outer_hist"] / cnt)
+        gw = self.giw_raw()
+        if gw.shape[1]:
+            # Giw * zScaling / beta (CTQMC.F90:8816) on the sums of the lookup
+            # path's -cfgsign M terms; n = -Niw..Niw-1, n < 0 the conjugates
+            pos = fl2(gw) / (cnt * mean_sign * beta)
+            out["giw"] = np.concatenate([np.conj(pos[..., ::-1]), pos], axis=-1)
+        return out

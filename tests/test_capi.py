@@ -140,6 +140,25 @@ def test_pooled_estimate_is_ratio_of_sums():
     assert np.allclose(r["gtau_err"], err.reshape(2, 1, ntau).swapaxes(0, 1))
 
 
+def test_pooled_giw_is_ratio_of_sums_with_complex_jackknife():
+    """G(i w_n) pooled: sum(raw) / (sum(sign) beta) for n >= 0, the conjugates for
+    n < 0 (Giw * zScaling / beta, CTQMC.F90:8816, on pooled sums), and a
+    jackknife error over the real and the imaginary part together."""
+    nb, ntau, nleg, beta, niw = 1, 4, 3, 2.0, 5
+    s = _synthetic_sums([-10.0, -6.0, -8.0, -9.0])
+    s["giw"] = np.random.default_rng(3).normal(size=(4, 2 * nb, niw, 2))
+    r = ctqmc.pooled_estimate(s, nb, beta, ntau, nleg, MODEL1)
+    raw = s["giw"][..., 0] + 1j * s["giw"][..., 1]
+    pos = raw.sum(0) / (s["scalars"][:, 0].sum() * beta)
+    assert r["giw"].shape == (nb, 2, 2 * niw)
+    assert np.allclose(r["giw"][..., niw:], pos.reshape(2, nb, niw).swapaxes(0, 1))
+    assert np.array_equal(r["giw"][..., :niw], np.conj(r["giw"][..., niw:][..., ::-1]))
+    jk = np.array([(raw.sum(0) - raw[i]) / ((s["scalars"][:, 0].sum() - s["scalars"][i, 0]) * beta)
+                   for i in range(4)])
+    err = np.sqrt(3 / 4 * (np.abs(jk - jk.mean(0)) ** 2).sum(0))
+    assert np.allclose(r["giw_err"][..., niw:], err.reshape(2, nb, niw).swapaxes(0, 1))
+
+
 def test_pooled_estimate_survives_a_zero_sign_chain():
     """One chain whose sign sum vanishes: its own normalisation is undefined
     (NaN, as a reference rank would divide by zero) but the pooled estimate and

@@ -228,6 +228,74 @@ def test_legendre_recursion_matches_numpy():
         plm1, pl = pl, plp1
 
 
+def legendre_to_matsubara(nleg: int, niw: int) -> np.ndarray:
+    """T[n, l] = (-1)^n i^(l+1) sqrt(2l+1) j_l((2n+1) pi / 2) for n = -niw..niw-1
+    (Boehnke et al. 2011, eq. E2; the transform w2dyn/auxiliaries/transform.py
+    leg2mat documents)."""
+    from scipy.special import spherical_jn
+    n = np.arange(-niw, niw)
+    l = np.arange(nleg)
+    z = (2 * n + 1) * np.pi / 2
+    # j_l(-x) = (-1)^l j_l(x)
+    jl = spherical_jn(l[None, :], np.abs(z)[:, None]) * np.sign(z)[:, None] ** l[None, :]
+    return ((-1.0) ** n)[:, None] * (1j ** (l + 1))[None, :] * np.sqrt(2 * l + 1)[None, :] * jl
+
+
+def test_giw_is_the_matsubara_transform_of_the_legendre_estimator():
+    """Every pair term of G(i w_n) is exp(i w_n dtau) and of G_l is P_l(x) on the
+    same M: by the plane-wave (Rayleigh) expansion exp(i z x) = sum_l (2l+1)
+    i^l j_l(z) P_l(x), G(i w_n) = sum_l T_nl G_l exactly once NLegMax exceeds
+    w_n beta / 2 by a margin.  A sign, conjugation, transposed M or shifted
+    frequency fails this."""
+    niw, nleg = 16, 120
+    p = P.kanamori_2orb(beta=10.0, MeasGiw=1, Niw=niw, NLegMax=nleg)
+    ch = OracleChain.from_problem(p, 7)
+    ch.warmup(2000)
+    ch.measure(300, 50)
+    r = ch.results()
+    giw = r["giw"]
+    assert giw.shape == (2, 2, 2 * niw)
+    want = np.einsum("nl,bsl->bsn", legendre_to_matsubara(nleg, niw), r["gleg"])
+    assert np.abs(giw - want).max() <= 1e-12 * np.abs(giw).max()
+    # real estimator: G(i w_{-n-1}) = conj G(i w_n)
+    assert np.array_equal(giw[..., :niw], np.conj(giw[..., niw:][..., ::-1]))
+
+
+def test_giw_frequency_grid_follows_the_nfft_fallback():
+    """One measurement of a fixed configuration against get_giw
+    (Nfft_z.F90:133-203) computed with the fallback adjoint transform
+    (Nfft_z.F90:64-90): 2 nfreq = 4 Niw frequencies on the grid
+    pi (2n - nn_total - 2), the shift exp(+-i pi tau / beta), every other output;
+    times -cfgsign (MeasGtauRem's lookup term) and zScaling / beta."""
+    niw, beta = 6, 20.0
+    p = P.kanamori_2orb(beta=beta, MeasGiw=1, Niw=niw)
+    ch = OracleChain.from_problem(p, 11)
+    ch.warmup(5000)
+    cfg = ch.config()
+    assert min(np.sum((cfg["orbs"] == b) & (cfg["spins"] == s)) for b in range(2) for s in range(2)) >= 4
+    ch.measure(1, 0)   # no moves: one measurement of this configuration
+    giw = ch.results()["giw"]
+    nfreq = 2 * niw
+    nn = 2 * nfreq
+    grid = np.pi * (2 * np.arange(1, nn + 1) - nn - 2)
+    cfgsign = cfg["cfgsign"]
+    for b in range(2):
+        for s in range(2):
+            f = b + 2 * s
+            sel = [(t, c) for t, o, sp, c in zip(cfg["taus"], cfg["orbs"], cfg["spins"], cfg["cas"])
+                   if o == b and sp == s]
+            tc = np.array([t for t, c in sel if c == 1])
+            ta = np.array([t for t, c in sel if c == 2])
+            m = ch.minv(f)   # rows creators, columns annihilators, time order
+            exp_c = np.exp(-1j * tc * np.pi / beta)
+            exp_a = np.exp(1j * ta * np.pi / beta)
+            func = (exp_c[None, :] * m.T * exp_a[:, None]).ravel()          # func(i, j)
+            x = ((ta[:, None] - tc[None, :]) / (2 * beta)).ravel()          # dtau(i, j)
+            fhat = (np.exp(1j * grid[:, None] * x[None, :]) * func[None, :]).sum(1)
+            ref = -cfgsign * fhat[0::2] * (-1.0 / (1 * cfgsign)) / beta
+            assert np.abs(giw[b, s] - ref).max() <= 1e-12 * max(np.abs(ref).max(), 1e-300)
+
+
 def _anderson_ed(eps_d, u, eps_k, v_k, beta):
     """Exact <n_up> and <n_up n_dn> of a single-orbital Anderson impurity with a
     discrete bath, from the full Fock space (bits: d_up, d_dn, then bath)."""

@@ -53,6 +53,11 @@ struct w2d_ctqmc {
     int *order = nullptr;      // device: chain of each workgroup
     bool order_chains = false, graphs = true;   // chain ordering: opt-in
     long long per_launch = 0;
+    // MeasGiw: raw G(i w_n) sums [nchains][2N][niw][2] and the per (chain,
+    // flavour) visit record of giw_kernel [nchains][2N][2]; niw = 0: off
+    int niw = 0;
+    double *giw = nullptr;
+    long long *giw_seen = nullptr;
     ~w2d_ctqmc();
 };
 
@@ -87,6 +92,7 @@ long align256(long x) { return (x + 255) / 256 * 256; }
 constexpr int KCAP_MAX = 4096;   // pairs per flavour a chain may grow to
 constexpr int ACCPAIR_BINS_MAX = 1024;   // accepted-pair histogram bins (NAccPair) at most, beyond Ntau
 constexpr int APT_CAP = 65536;           // accepted-pair distances a calibration list holds at most
+constexpr int GIW_NIW_MAX = 1 << 20;     // Matsubara frequencies of MeasGiw at most
 
 // Static device scratch of the allocation-free entry point (16 MiB per GPU,
 // held until the process ends).  In host code `g_scratch` names a host-side
@@ -420,6 +426,14 @@ void launch_sweep(w2d_ctqmc *h, long long target, long long ncorr, int only, w2d
         hipLaunchKernelGGL((w2d::sweep_kernel<NM, false>), dim3(h->nchains), dim3(w2d::WAVE), (unsigned)h->L.lds_bytes,
                            h->stream, h->dm, h->L, h->chains, target, ncorr, only, rec, rec_base,
                            (const int *)h->abort_dev, h->calib, order);
+    // G(i w) of the configurations this launch measured (launches end on a
+    // measurement when MeasGiw is on, see launch_range)
+    if (h->niw > 0 && ncorr > 0) {
+        const int per_thread = (h->niw + w2d::GIW_R - 1) / w2d::GIW_R;
+        const int nt = std::min(w2d::GIW_THREADS, (per_thread + 63) / 64 * 64);
+        hipLaunchKernelGGL(w2d::giw_kernel, dim3(h->nchains, h->dm.nf), dim3(nt), 0, h->stream, h->L,
+                           (const char *)h->chains, h->dm.nf, h->niw, h->beta, h->giw, h->giw_seen);
+    }
 }
 
 template <int NM>
@@ -543,8 +557,11 @@ int map_err(int e)
 void launch_range(w2d_ctqmc *h, long long from, long long target, long long ncorr, long long per, int only,
                   w2d_move_record *rec, long long rec_base)
 {
+    // with MeasGiw every launch ends on a measurement: targets on multiples of
+    // ncorr, also when a resumed pass starts between two of them
+    const bool giw = h->niw > 0 && ncorr > 0;
     for (long long t = from; t < target;) {
-        t = std::min(target, t + per);
+        t = std::min(target, giw ? (t / ncorr + 1) * ncorr : t + per);
         DISPATCH(launch_sweep, h, t, ncorr, only, rec, rec_base);
         ++h->last_launches;
     }
@@ -600,6 +617,7 @@ int run_moves(w2d_ctqmc *h, long long nmoves, long long ncorr)
     h->aborted = 0;
     const long long base = h->per_launch > 0 ? h->per_launch : 4000;
     long long per = ncorr > 0 ? ncorr * std::max<long long>(1, base / ncorr) : base;
+    if (h->niw > 0 && ncorr > 0) per = ncorr;   // one measurement per launch (MeasGiw)
     h->last_launches = 0;
     h->last_ms = 0.0;
     std::vector<ChainHdr> hd;
@@ -812,7 +830,7 @@ int w2d_ctqmc_create(const char *params, int nbands, int nftau, const double *u_
     // measurements this library does not produce: refused rather than left
     // unmeasured (the caller would read arrays that were never filled)
     {
-        const char *off_keys[] = {"MeasGiw", "MeasG2iw", "MeasGSigmaiw", "FourPnt", "MeasSusz", "MeasSsquare",
+        const char *off_keys[] = {"MeasG2iw", "MeasGSigmaiw", "FourPnt", "MeasSusz", "MeasSsquare",
                                   "MeasExpResDensityMatrix", "MeasGtauDetRat", "WormMeasGiw", "WormMeasGtau"};
         for (const char *k : off_keys)
             if (P.has(k) && P.integer(k) != 0) {
@@ -972,6 +990,25 @@ int w2d_ctqmc_create(const char *params, int nbands, int nftau, const double *u_
     d.maxhisto = h->maxhisto;
     d.has_szt = P.str("QuantumNumbers").find("Szt") != std::string::npos;
     d.meas_dm = P.integer("MeasDensityMatrix") != 0;
+    // MeasGiw with Niw frequencies (configspec default 2000)
+    if (P.has("MeasGiw") && P.integer("MeasGiw") != 0) {
+        const long long niw = P.has("Niw") ? P.integer("Niw") : 2000;
+        if (niw < 1 || niw > GIW_NIW_MAX) {
+            std::fprintf(stderr, "w2d_ctqmc: Niw = %lld outside [1, %d]\n", niw, GIW_NIW_MAX);
+            return W2D_ERR_ARG;
+        }
+        h->niw = (int)niw;
+        const size_t nsum = (size_t)nchains * nf * (size_t)niw * 2, nseen = (size_t)nchains * nf * 2;
+        void *a = nullptr, *b = nullptr;
+        HIPCHK(hipMalloc(&a, nsum * sizeof(double)));
+        h->dev_allocs.push_back(a);
+        HIPCHK(hipMalloc(&b, nseen * sizeof(long long)));
+        h->dev_allocs.push_back(b);
+        h->giw = static_cast<double *>(a);
+        h->giw_seen = static_cast<long long *>(b);
+        HIPCHK(hipMemset(a, 0, nsum * sizeof(double)));
+        HIPCHK(hipMemset(b, 0, nseen * sizeof(long long)));
+    }
     // AccumGtau: 1 meanacc, 2 autocorracc with ilog2(Nmeas) levels (CTQMC.F90:491-503)
     d.accum = P.has("AccumGtau") ? (int)P.integer("AccumGtau") : 1;
     if (d.accum != 1 && d.accum != 2) return W2D_ERR_ARG;
@@ -1050,6 +1087,11 @@ int w2d_ctqmc_measure(w2d_ctqmc *h, int64_t nmeas)
     const w2d::ChainLayout &L = h->L;
     HIPCHK(hipMemset2DAsync(h->chains + L.gtau, (size_t)L.stride, 0, (size_t)(L.stride - L.gtau),
                             (size_t)h->nchains, h->stream));
+    if (h->niw > 0) {
+        const size_t nf = (size_t)h->dm.nf, nc = (size_t)h->nchains;
+        HIPCHK(hipMemsetAsync(h->giw, 0, nc * nf * (size_t)h->niw * 2 * sizeof(double), h->stream));
+        HIPCHK(hipMemsetAsync(h->giw_seen, 0, nc * nf * 2 * sizeof(long long), h->stream));
+    }
     st = run_moves(h, nmeas * h->ncorr, h->ncorr);
     h->measured = (st == W2D_OK || st == W2D_ERR_PARTIAL);
     return st;
@@ -1462,6 +1504,50 @@ int w2d_ctqmc_get_group_sums_noalloc(w2d_ctqmc *h, int ngroups, double *gtau, do
         return W2D_ERR_CAPACITY;
     }
     return group_sums_into(h, ngroups, h->scratch, gtau, gleg, dm_blocks, histo, outer, scalars, nused);
+}
+
+int w2d_ctqmc_get_giw(w2d_ctqmc *h, int ngroups, int *niw, double *out)
+{
+    if (!h || !niw || ngroups < 0 || ngroups > h->nchains) return W2D_ERR_ARG;
+    *niw = h->niw;
+    if (!out) return W2D_OK;
+    if (h->niw == 0 || !h->measured) return W2D_ERR_STATE;
+    const int nf = h->dm.nf, nc = h->nchains;
+    const size_t per = (size_t)nf * h->niw * 2;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    std::vector<double> all((size_t)nc * per);
+    std::vector<long long> seen((size_t)nc * nf * 2);
+    HIPCHK(hipMemcpy(all.data(), h->giw, all.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(seen.data(), h->giw_seen, seen.size() * sizeof(long long), hipMemcpyDeviceToHost));
+    std::vector<ChainHdr> hd;
+    int st = read_headers(h, hd);
+    if (st) return st;
+    // every measurement of a chain that completed has its G(i w) term
+    for (int c = 0; c < nc; ++c) {
+        if (hd[c].err) continue;
+        for (int f = 0; f < nf; ++f)
+            if (seen[((size_t)c * nf + f) * 2 + 1] != hd[c].cnt_meas) {
+                std::fprintf(stderr, "w2d_ctqmc: chain %d flavour %d has %lld G(iw) terms for %lld measurements\n", c,
+                             f, seen[((size_t)c * nf + f) * 2 + 1], hd[c].cnt_meas);
+                return W2D_ERR_STATE;
+            }
+    }
+    if (ngroups == 0) {
+        std::copy(all.begin(), all.end(), out);
+        return W2D_OK;
+    }
+    // groups of contiguous chains as in w2d_ctqmc_get_group_sums
+    std::fill(out, out + (size_t)ngroups * per, 0.0);
+    for (int g = 0; g < ngroups; ++g) {
+        const int c0 = (int)((long)g * nc / ngroups), c1 = (int)((long)(g + 1) * nc / ngroups);
+        double *o = out + (size_t)g * per;
+        for (int c = c0; c < c1; ++c) {
+            if (hd[c].err) continue;
+            const double *a = all.data() + (size_t)c * per;
+            for (size_t i = 0; i < per; ++i) o[i] += a[i];
+        }
+    }
+    return W2D_OK;
 }
 
 int w2d_ctqmc_get_gtau_blocks(w2d_ctqmc *h, int chain, int *nlevels, double *blocks)

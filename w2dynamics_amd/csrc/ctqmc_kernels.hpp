@@ -3619,6 +3619,152 @@ __global__ void __launch_bounds__(1024) order_kernel(const ChainHdr *hdrs, int n
     for (int c = t; c < nc; c += 1024) order[atomicAdd(&cnt[key(c)], 1)] = c;
 }
 
+// MeasGiw (measure_giw_nfft, CTQMC.F90:1015-1059; the sum is the direct one of
+// the NFFT fallback, Nfft_z.F90:64-90).  After a measurement, for every
+// flavour f and n = 0..niw-1:
+//   giw[chain][f][n] += cfgsign * sum_{a,c} M[c,a] exp(i w_n (tau_a - tau_c)),
+// w_n = (2n+1) pi / beta, M the inverse hybridisation block (rows creators,
+// columns annihilators, by physical slot), with the sign convention of the
+// lookup path (CTQMC.F90:2403-2417 and :8816); negative frequencies are the
+// complex conjugates.  The kernel follows every sweep launch of whole
+// measurement intervals (the chain's state at the end of such a launch is the
+// state it measured) and acts on the chains whose measurement count moved
+// since its last visit: seen[chain][f] = {count at the last visit, G(i w)
+// measurements made}; the host checks that the second equals the count.
+// One workgroup per (chain, flavour).  A thread takes GIW_R consecutive
+// frequencies; creators go in tiles of GIW_TC whose phases stay in registers,
+// annihilators in tiles of GIW_TA with the M tile in LDS; the phase of the
+// next frequency is one complex product with exp(2 pi i tau / beta).
+constexpr int GIW_THREADS = 256, GIW_R = 4, GIW_TC = 8, GIW_TA = 64;
+
+__global__ void __launch_bounds__(GIW_THREADS) giw_kernel(ChainLayout L, const char *chains, int nf, int niw,
+                                                         double beta, double *giw, long long *seen)
+{
+    __shared__ double s_m[GIW_TA][GIW_TC];
+    __shared__ double s_ct[GIW_TC], s_at[GIW_TA];
+    __shared__ double2 s_cs[GIW_TC], s_as[GIW_TA];
+    __shared__ int s_bad;
+    const int chain = blockIdx.x, f = blockIdx.y, tid = threadIdx.x, nt = blockDim.x;
+    // header fields written by the sweep kernels: device-coherent loads
+    const ChainHdr *hp = reinterpret_cast<const ChainHdr *>(chains + L.hdrs) + chain;
+    const long long cnt = __hip_atomic_load(&hp->cnt_meas, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int err = __hip_atomic_load(&hp->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    long long *sv = seen + 2 * ((long)chain * nf + f);
+    const long long last = sv[0];
+    if (err != 0 || cnt == last) return;   // uniform over the workgroup
+    const int k = __hip_atomic_load(&hp->kf[f], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const double sg = __longlong_as_double(__hip_atomic_load(reinterpret_cast<const long long *>(&hp->cfgsign),
+                                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    if (tid == 0) s_bad = (k < 0 || k > L.kcap) ? 1 : 0;
+    __syncthreads();
+    const int kk = s_bad ? 0 : k;
+    const char *base = chains + (long)chain * L.stride;
+    const double *tcl = reinterpret_cast<const double *>(base + L.fc) + (long)f * L.kcap;
+    const double *tal = reinterpret_cast<const double *>(base + L.fa) + (long)f * L.kcap;
+    const int *csl = reinterpret_cast<const int *>(base + L.fcs) + (long)f * L.kcap;
+    const int *asl = reinterpret_cast<const int *>(base + L.fas) + (long)f * L.kcap;
+    const double *Mf = reinterpret_cast<const double *>(base + L.minv) + (long)f * L.kcap * L.kcap;
+    double *out = giw + ((long)chain * nf + f) * (long)niw * 2;
+    const double ib = 1.0 / beta;
+    for (int n0 = 0; n0 < niw; n0 += nt * GIW_R) {
+        const int nb = n0 + tid * GIW_R;            // first frequency of this thread
+        const double xb = 2.0 * nb + 1.0;           // w_nb tau / pi = xb * tau / beta
+        double gr[GIW_R], gi[GIW_R];
+#pragma unroll
+        for (int r = 0; r < GIW_R; ++r) gr[r] = gi[r] = 0.0;
+        for (int c0 = 0; c0 < kk; c0 += GIW_TC) {
+            const int nc = min(GIW_TC, kk - c0);
+            __syncthreads();
+            if (tid < GIW_TC) {
+                const double t = tid < nc ? tcl[c0 + tid] * ib : 0.0;
+                double s, c;
+                sincospi(2.0 * t, &s, &c);
+                s_ct[tid] = t;
+                s_cs[tid] = make_double2(c, s);
+            }
+            __syncthreads();
+            // exp(-i w_n tau_c) for the thread's frequencies; zero past the tile
+            double cr[GIW_R][GIW_TC], ci[GIW_R][GIW_TC];
+#pragma unroll
+            for (int j = 0; j < GIW_TC; ++j) {
+                double s, c;
+                sincospi(xb * s_ct[j], &s, &c);
+                const double2 st = s_cs[j];
+                const double on = j < nc ? 1.0 : 0.0;
+                double pr = c * on, pi = -s * on;
+#pragma unroll
+                for (int r = 0; r < GIW_R; ++r) {
+                    cr[r][j] = pr;
+                    ci[r][j] = pi;
+                    const double t = pr * st.x + pi * st.y;
+                    pi = pi * st.x - pr * st.y;
+                    pr = t;
+                }
+            }
+            for (int a0 = 0; a0 < kk; a0 += GIW_TA) {
+                const int na = min(GIW_TA, kk - a0);
+                __syncthreads();
+                for (int e = tid; e < GIW_TA * GIW_TC; e += nt) {
+                    const int ai = e / GIW_TC, cj = e - ai * GIW_TC;
+                    double v = 0.0;
+                    if (ai < na && cj < nc) {
+                        const int rs = csl[c0 + cj], cs = asl[a0 + ai];
+                        if (rs >= 0 && rs < L.kcap && cs >= 0 && cs < L.kcap) v = Mf[rs + (long)cs * L.kcap];
+                        else s_bad = 1;
+                    }
+                    s_m[ai][cj] = v;
+                }
+                for (int ai = tid; ai < GIW_TA; ai += nt) {
+                    const double t = ai < na ? tal[a0 + ai] * ib : 0.0;
+                    double s, c;
+                    sincospi(2.0 * t, &s, &c);
+                    s_at[ai] = t;
+                    s_as[ai] = make_double2(c, s);
+                }
+                __syncthreads();
+                for (int ai = 0; ai < na; ++ai) {
+                    double ur[GIW_R], ui[GIW_R];
+#pragma unroll
+                    for (int r = 0; r < GIW_R; ++r) ur[r] = ui[r] = 0.0;
+#pragma unroll
+                    for (int j = 0; j < GIW_TC; ++j) {
+                        const double m = s_m[ai][j];
+#pragma unroll
+                        for (int r = 0; r < GIW_R; ++r) {
+                            ur[r] = fma(m, cr[r][j], ur[r]);
+                            ui[r] = fma(m, ci[r][j], ui[r]);
+                        }
+                    }
+                    double s, c;
+                    sincospi(xb * s_at[ai], &s, &c);
+                    const double2 st = s_as[ai];
+#pragma unroll
+                    for (int r = 0; r < GIW_R; ++r) {
+                        gr[r] += c * ur[r] - s * ui[r];
+                        gi[r] += c * ui[r] + s * ur[r];
+                        const double t = c * st.x - s * st.y;
+                        s = c * st.y + s * st.x;
+                        c = t;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < GIW_R; ++r) {
+            const int n = nb + r;
+            if (n < niw) {
+                out[2 * n] += sg * gr[r];
+                out[2 * n + 1] += sg * gi[r];
+            }
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        sv[0] = cnt;
+        if (!s_bad && cnt == last + 1) sv[1] += 1;
+    }
+}
+
 __global__ void __launch_bounds__(256) group_sum_kernel(const char *chains, long stride, long acc_off, long nel,
                                                         int nc, int ng, const ChainHdr *hdrs, double *out)
 {

@@ -84,6 +84,7 @@ def load_library(path: str | None = None):
     lib.w2d_ctqmc_aborted.argtypes = [P, C.POINTER(I)]
     lib.w2d_ctqmc_trap_signals.argtypes = []
     lib.w2d_ctqmc_get_gtau_blocks.argtypes = [P, I, C.POINTER(I), P]
+    lib.w2d_ctqmc_get_giw.argtypes = [P, I, C.POINTER(I), P]
     lib.w2d_ctqmc_get_orders.argtypes = [P, P, P]
     lib.w2d_ctqmc_op_table_bytes.argtypes = [P, C.POINTER(C.c_int64)]
     lib.w2d_ctqmc_init_counters.argtypes = [P]
@@ -356,7 +357,30 @@ class CtqmcGpu:
                                                    _p(sc)), "get_accumulators")
         # dm rows are column-major per chain
         dm = np.transpose(dm, (0, 2, 1))
-        return dict(gtau=g, gleg=gl, dm=dm, histo=hi, outer_hist=oh, scalars=sc)
+        out = dict(gtau=g, gleg=gl, dm=dm, histo=hi, outer_hist=oh, scalars=sc)
+        if self.niw:
+            out["giw"] = self.giw_sums(0)
+        return out
+
+    @property
+    def niw(self) -> int:
+        """Matsubara frequencies of MeasGiw (0: the handle does not measure G(i w))."""
+        n = C.c_int()
+        _check(self.lib.w2d_ctqmc_get_giw(self.h, 0, C.byref(n), None), "get_giw")
+        return n.value
+
+    def giw_sums(self, ngroups: int = 0) -> np.ndarray:
+        """MeasGiw: raw sums of cfgsign M(c,a) exp(i w_n (tau_a - tau_c)) for
+        n = 0..Niw-1, [chain or group][flavour][n][re, im]; ngroups = 0 per chain,
+        else over the groups of ``group_sums``."""
+        n = C.c_int()
+        _check(self.lib.w2d_ctqmc_get_giw(self.h, 0, C.byref(n), None), "get_giw")
+        if n.value == 0:
+            raise CtqmcError(6, "get_giw (the handle was created without MeasGiw)")
+        rows = self.nchains if ngroups == 0 else int(ngroups)
+        out = np.zeros((rows, 2 * self.nbands, n.value, 2))
+        _check(self.lib.w2d_ctqmc_get_giw(self.h, int(ngroups), C.byref(n), _p(out)), "get_giw")
+        return out
 
     def results(self, acc: dict | None = None) -> dict:
         """Per-chain observables normalised as in ctqmc_measure (each chain like
@@ -393,6 +417,8 @@ class CtqmcGpu:
         _check(self.lib.w2d_ctqmc_get_group_sums(self.h, ng, _p(g), _p(gl), _p(db) if with_dm else None,
                                                  _p(hi), _p(oh), _p(sc), _p(nu)), "get_group_sums")
         out = dict(gtau=g, gleg=gl, histo=hi, outer_hist=oh, scalars=sc, nused=nu)
+        if self.niw:
+            out["giw"] = self.giw_sums(ng)
         if with_dm:
             dm = np.zeros((ng, self.nstates, self.nstates))
             o = 0
@@ -553,7 +579,7 @@ def pooled_estimate(sums: dict, nb: int, beta: float, ntau: int, nleg: int, mode
     """Normalised observables of the pooled sums over the leading (group) axis
     of ``sums`` and their jackknife errors over that axis."""
     keys = [k for k in ("gtau", "gleg", "dm", "histo", "outer_hist", "scalars", "outer_state", "sign_sst",
-                        "sign_state", "contrib_sst", "contrib_state", "accpair") if k in sums]
+                        "sign_state", "contrib_sst", "contrib_state", "accpair", "giw") if k in sums]
     tot = {k: np.asarray(sums[k], dtype=float).sum(0)[None] for k in keys}
     est = normalise(tot, nb, beta, ntau, nleg, model)
     out = {k: np.asarray(v)[0] for k, v in est.items()}
@@ -569,10 +595,10 @@ def pooled_estimate(sums: dict, nb: int, beta: float, ntau: int, nleg: int, mode
         part = {k: tot[k] - np.asarray(sums[k], dtype=float)[i][None] for k in keys}
         r = normalise(part, nb, beta, ntau, nleg, model)
         for k in est:
-            jk[k].append(np.asarray(r[k], dtype=float)[0])
+            jk[k].append(np.asarray(r[k])[0])
     for k in est:
         a = np.array(jk[k])
-        out[k + "_err"] = np.sqrt((n - 1) / n * ((a - a.mean(0)) ** 2).sum(0))
+        out[k + "_err"] = np.sqrt((n - 1) / n * (np.abs(a - a.mean(0)) ** 2).sum(0))
     out["nchains_groups"] = n
     return out
 
@@ -612,6 +638,13 @@ def _normalise(acc, nb, beta, ntau, nleg, model):
                histo=fl2(acc["histo"]) / cnt[:, None, None, None], acc=accr,
                outer_sst_histo=acc["outer_hist"] / cnt[:, None])
     out.update(normalise_histograms(acc, cnt, nb, beta))
+    if "giw" in acc:
+        # Giw * zScaling / beta (CTQMC.F90:8816) of the lookup path's -cfgsign M
+        # sums: sum / (count x mean sign x beta); n = -Niw..Niw-1, n < 0 the
+        # complex conjugates of n' = -n-1
+        raw = np.asarray(acc["giw"], dtype=float)
+        pos = fl2(raw[..., 0] + 1j * raw[..., 1]) / (cnt * mean_sign * beta)[:, None, None, None]
+        out["giw"] = np.concatenate([np.conj(pos[..., ::-1]), pos], axis=-1)
     return out
 
 
