@@ -65,7 +65,15 @@ z = load_golden(sys.argv[1])
 nb = int(z["nbands"])
 n2 = 2 * nb
 ftau = np.asfortranarray(z["ftau"], dtype=np.float64)
-m.init_paras(str(z["param_string"]))
+params = str(z["param_string"])
+giw_mode = len(sys.argv) > 3 and sys.argv[3] == "giw"
+if giw_mode:
+    # MeasGiw with 64 frequencies (result["giw-meas"] = ctqmc.giw, cthyb_solver.py:989-990)
+    lines = params.split("\n")
+    lines = ["MeasGiw = 1 #" if l == "MeasGiw = 0 #" else "Niw = 64 #" if l.startswith("Niw =") else l
+             for l in lines]
+    params = "\n".join(lines)
+m.init_paras(params)
 m.init_rnd(int(z["seed"]))
 m.init_solver(np.asfortranarray(np.asarray(z["umatrix"]).reshape(n2, n2, n2, n2)), ftau,
               np.asfortranarray(z["muimp"], dtype=np.float64), np.zeros_like(ftau), 1)
@@ -87,6 +95,8 @@ norb = result["gtau"].shape[0]
 if "rho2" in result:
     result["rho2"] = result["rho2"].reshape((norb, 2) * 4)
     result["rho1"] = result["rho1"].reshape((norb, 2) * 2)
+if giw_mode:
+    result["giw-meas"] = np.copy(m.giw)
 for k, v in result.items():
     out["rs_" + k] = np.asarray(v)
 if len(sys.argv) > 3 and sys.argv[3] == "reuse":
@@ -173,6 +183,34 @@ def test_f2py_module_reuses_the_mc_configuration(tmp_path):
         assert np.array_equal(got["reuse_set_" + k], got["reuse_back_" + k]), k
     assert np.all(got["reuse_set_hashybs"] == -1)
     assert np.all(np.isfinite(got["reuse_gtau"]))
+
+
+@pytest.mark.gpu
+def test_f2py_module_measures_giw(tmp_path):
+    """MeasGiw through the extension: ``giw`` (NBands, 2, 2 Niw) of the golden
+    run equals the oracle's G(i w_n) of the same Markov chain."""
+    import os
+    import subprocess
+    import sys
+    from conftest import ROOT
+    from oracle import OracleChain
+    z = load_golden("c2_run")
+    script = DRIVE % dict(root=ROOT, tests=os.path.join(ROOT, "tests"), resultset=RESULTSET, density=DENSITY)
+    out = str(tmp_path / "out.npz")
+    print("f2py module giw", flush=True)
+    r = subprocess.run([sys.executable, "-c", script, "c2_run", out, "giw"], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    got = np.load(out)["rs_giw-meas"]
+    q = qmc_of(z)
+    q["MeasGiw"], q["Niw"] = "1", "64"
+    ch = OracleChain(z["umatrix"], z["ftau"], z["muimp"], float(z["beta"]), q, int(z["seed"]))
+    ch.set_taudiffmax(float(q["TaudiffMax"]))
+    ch.warmup(int(z["nwarm"]))
+    ch.measure(int(q["Nmeas"]), int(q["NCorr"]))
+    want = ch.results()["giw"]
+    assert got.shape == want.shape == (int(z["nbands"]), 2, 128)
+    assert np.abs(got - want).max() <= 1e-9 * np.abs(want).max()
 
 
 def test_f2py_module_reports_library_errors(tmp_path):

@@ -76,6 +76,9 @@ module MCTQMC
    !> AccumGtau = 2: binning variances of the raw G(tau) per level
    !! (gtau_blocks, CTQMC.F90:8733-8739), mean over chains
    real(KINDR), allocatable :: gtau_blocks(:,:,:,:)
+   !> MeasGiw: G(i w_n), n = -Niw..Niw-1 (Giw, CTQMC.F90:573, normalised as
+   !! 8816) from the sums over all chains; allocated if enabled
+   complex(KINDR), allocatable :: Giw(:,:,:)        ! (NBands,2,-Niw:Niw-1)
    real(KINDR) :: time_calibration = 0
 
    type(c_ptr), private, save :: handle = c_null_ptr
@@ -90,10 +93,11 @@ module MCTQMC
    ! interface (an f2py wrapper of the module sees the reference's names only)
    private :: failed, param_value, load_model, collect_accpair, sort_ascending, collect_results, &
               collect_histograms, apply_ops, reduced_density_matrices, collect_steps_and_blocks, &
-              reset_results
+              reset_results, collect_giw
    private :: w2d_ctqmc_aborted, w2d_ctqmc_calibrate, w2d_ctqmc_create, w2d_ctqmc_destroy, &
               w2d_ctqmc_dims, w2d_ctqmc_get_accumulators, w2d_ctqmc_get_config, &
               w2d_ctqmc_get_group_histograms, w2d_ctqmc_get_group_sums, w2d_ctqmc_get_gtau_blocks, &
+              w2d_ctqmc_get_giw, &
               w2d_ctqmc_get_model, w2d_ctqmc_get_pair_stats, w2d_ctqmc_get_steps, &
               w2d_ctqmc_init_counters, w2d_ctqmc_measure, w2d_ctqmc_set_config, &
               w2d_ctqmc_set_taudiffmax, w2d_ctqmc_trap_signals, w2d_ctqmc_warmup
@@ -216,6 +220,13 @@ module MCTQMC
          integer(c_int), value :: chain
          integer(c_int) :: nl
          type(c_ptr), value :: blocks      ! c_null_ptr: size only
+      end function
+      integer(c_int) function w2d_ctqmc_get_giw(h, ngroups, niw, out) bind(C, name="w2d_ctqmc_get_giw")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: h
+         integer(c_int), value :: ngroups
+         integer(c_int) :: niw
+         type(c_ptr), value :: out         ! c_null_ptr: size only
       end function
       integer(c_int) function w2d_ctqmc_destroy(h) bind(C, name="w2d_ctqmc_destroy")
          import :: c_int, c_ptr
@@ -531,6 +542,7 @@ contains
       call collect_accpair()
       call reduced_density_matrices()
       call collect_steps_and_blocks()
+      call collect_giw(cnt, msign)
       ! occ(b1,s1,b2,s2) = sum_k rho_kk n_{b1 s1}(k) n_{b2 s2}(k), occupation basis
       do k = 1, nstates
          diag(k) = DensityMatrix(k, k)
@@ -670,6 +682,30 @@ contains
          end do
       end if
    end subroutine collect_steps_and_blocks
+
+   !> MeasGiw: Giw * zScaling / beta (CTQMC.F90:8816) of the sums over the
+   !! chains that completed, i.e. sum / (count x mean sign x beta); the
+   !! negative frequencies are the complex conjugates
+   subroutine collect_giw(cnt, msign)
+      real(KINDR), intent(in) :: cnt, msign
+      integer(c_int) :: niw
+      integer :: b, s, f, n
+      real(KINDR), allocatable, target :: g(:,:,:)
+      if (allocated(Giw)) deallocate (Giw)
+      if (failed(w2d_ctqmc_get_giw(handle, 0_c_int, niw, c_null_ptr), "w2d_ctqmc_get_giw")) return
+      if (niw == 0) return
+      allocate (g(2, niw, 2*nb), Giw(nb, 2, -niw:niw - 1))
+      if (failed(w2d_ctqmc_get_giw(handle, 1_c_int, niw, c_loc(g)), "w2d_ctqmc_get_giw")) return
+      do s = 1, 2
+         do b = 1, nb
+            f = b + nb*(s - 1)
+            do n = 0, niw - 1
+               Giw(b, s, n) = cmplx(g(1, n + 1, f), g(2, n + 1, f), KINDR)/(cnt*msign*beta)
+               Giw(b, s, -n - 1) = conjg(Giw(b, s, n))
+            end do
+         end do
+      end do
+   end subroutine collect_giw
 
    subroutine reset_results()
       if (allocated(Gtau)) deallocate (Gtau, GLeg, occ, DensityMatrix, Histo, OuterSuperstateHisto)
