@@ -143,10 +143,10 @@ def thermalise(gpu, ncorr, min_sweeps, chunk, budget_s, tol=0.015):
     return traj
 
 
-def run_gpu(cfg, nchains, capacity, min_sweeps, steps, ncorr, seed, therm_s, launch=None, engine=0):
+def run_gpu(cfg, nchains, capacity, min_sweeps, steps, ncorr, seed, therm_s, launch=None, engine=0, niw=0):
     import torch
     from w2dynamics_amd.ctqmc import CtqmcGpu, check_accumulators
-    prob, tdm = make_problem(cfg, ncorr)
+    prob, tdm = make_problem(cfg, ncorr, **({"MeasGiw": 1, "Niw": niw} if niw else {}))
     gpu = CtqmcGpu.from_problem(prob, nchains=nchains, seed=seed, capacity=capacity, device=0)
     gpu.set_launch(**(launch or {}))
     if engine > 0:
@@ -169,7 +169,21 @@ def run_gpu(cfg, nchains, capacity, min_sweeps, steps, ncorr, seed, therm_s, lau
     check_accumulators(sums)       # raises if any G(tau) accumulation was lost
     sc = sums["scalars"]
     q = prob.qmc()
-    out = dict(wall=wall, kernel_ms=kms, launches=nlaunch,
+    giw_flops = None
+    if niw:
+        # G(i w) kernel work per launch, from the configurations at the end:
+        # per flavour k^2 complex-by-real products (4 flops) and k complex
+        # products (8 flops) for every frequency
+        fl = 0.0
+        for c in range(nchains):
+            x = gpu.get_config(c)
+            for b in range(prob.nbands):
+                for s in range(2):
+                    k = int(((x["orbs"] == b) & (x["spins"] == s)).sum()) // 2
+                    fl += (4.0 * k * k + 8.0 * k) * niw
+        giw_flops = fl
+        assert np.isfinite(gpu.results()["giw"]).all()
+    out = dict(wall=wall, kernel_ms=kms, launches=nlaunch, giw_flops=giw_flops,
                inv_err2=ratio_inv_err2(sums["gtau"], sc[:, 0], int(q["Ntau"]), prob.beta),
                moves=float(sc[:, 10].sum()), alg_flops=float(sc[:, 12].sum()),
                alg_bytes=float(sc[:, 13].sum()), mean_sign=float(sc[:, 0].sum() / sc[:, 1].sum()),
@@ -457,6 +471,10 @@ def main():
                     "-1: autotune")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--c4-anyway", action="store_true", help="run the 5-orbital workload although it is switched off")
+    ap.add_argument("--niw", type=int, default=0,
+                    help="also measure G(i w_n) (MeasGiw) on this many frequencies: measure launches end on every "
+                         "measurement and the G(iw) kernel follows each; the roofline is then left out (the "
+                         "launches time both kernels)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run 5 device copies of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -489,7 +507,8 @@ def main():
     try:
         b = run_gpu(cfg, nchains, capacity, args.warmup, args.steps * SWEEPS_PER_STEP[cfg], args.ncorr, seed,
                     args.therm_s,
-                    launch=dict(graphs=not args.no_graphs, order_chains=args.order), engine=args.engine)
+                    launch=dict(graphs=not args.no_graphs, order_chains=args.order), engine=args.engine,
+                    niw=args.niw)
     except BaseException:
         if cpu_run:
             cpu_run.cancel()
@@ -516,6 +535,13 @@ def main():
             "alg_flops_per_launch": b["alg_flops"] / launches, "alg_bytes_per_launch": alg_bytes / launches,
             "fp64_tflops": ach_tfs, "fp64_frac": ach_tfs / FP64_PEAK_TFS,
             "hbm_gbs": ach_gbs, "hbm_frac": ach_gbs / HBM_PEAK_GBS}
+    meas_giw = None
+    if args.niw:
+        meas_giw = {"niw": args.niw, "launches": b["launches"], "kernel": "giw_kernel",
+                    "alg_flops_per_launch": b["giw_flops"],
+                    "note": "the timed region and the launch times include the G(iw) kernel after every "
+                            "measurement; per-kernel times come from rocprofv3"}
+        roof = None
 
     plain = None
     if not args.no_plain:
@@ -562,6 +588,8 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    if meas_giw:
+        line["meas_giw"] = meas_giw
     print(json.dumps(line))
 
 
