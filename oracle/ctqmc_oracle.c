@@ -355,6 +355,10 @@ static olog_t local_trace(ochain_t *ch, olog_t *part)
         for (int j = 0; j < ch->nop; ++j) {
             const oop_t *o = &ch->op[j];
             int t = conn(ch, s, o->orb, o->spin, o->ca);
+            if (t < 0) {   /* no Connect target: the outer superstate's trace is zero */
+                s = -1;
+                break;
+            }
             int nt = ch->sizes[t];
             const double *E = ch->evals + ch->offsets[s];
             double dt = o->tau - tr0;
@@ -376,6 +380,13 @@ static olog_t local_trace(ochain_t *ch, olog_t *part)
             s = t;
             ns = nt;
             tr0 = o->tau;
+        }
+        if (s != outer) {
+            /* a configuration that breaks the quantum numbers (set from
+             * outside; the moves never propose one) has zero weight */
+            if (part)
+                for (int q = 0; q < no; ++q) part[q] = (olog_t){-DBL_MAX, 0.0};
+            return (olog_t){-DBL_MAX, 0.0};
         }
         /* propagate to beta and project on <i0| */
         const double *E = ch->evals + ch->offsets[s];

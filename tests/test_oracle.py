@@ -170,6 +170,17 @@ def test_two_operator_trace_closed_form():
                 assert abs(w["trace_sign"] * math.exp(w["trace_log"]) - val) <= 1e-12 * abs(val)
 
 
+def test_configuration_breaking_quantum_numbers_has_zero_weight():
+    """Two creators of one flavour in a row have no Connect target on any outer
+    superstate: the trace is zero (log -huge, sign 0), not a read past the
+    block tables."""
+    p = P.kanamori_2orb(beta=8.0)
+    ch = OracleChain.from_problem(p, 3)
+    for s in range(ch.model.nsst):
+        w = ch.set_config([1.0, 2.0, 3.0, 4.0], [0, 0, 0, 0], [0, 0, 0, 0], [1, 1, 2, 2], s)
+        assert w["trace_sign"] == 0.0 and w["trace_log"] < -1e300
+
+
 def test_determinant_and_inverse_brute_force():
     """det and M = Delta^{-1} of a visited configuration against numpy."""
     p = P.kanamori_2orb(beta=20.0, TaudiffMax=4.0)
