@@ -171,14 +171,18 @@ def test_two_operator_trace_closed_form():
 
 
 def test_configuration_breaking_quantum_numbers_has_zero_weight():
-    """Two creators of one flavour in a row have no Connect target on any outer
-    superstate: the trace is zero (log -huge, sign 0), not a read past the
-    block tables."""
+    """An annihilator acting first on the empty superstate has no Connect target:
+    the walk stops with a zero trace instead of reading past the block tables,
+    and the same pair on a superstate that has the electron stays finite."""
     p = P.kanamori_2orb(beta=8.0)
     ch = OracleChain.from_problem(p, 3)
-    for s in range(ch.model.nsst):
-        w = ch.set_config([1.0, 2.0, 3.0, 4.0], [0, 0, 0, 0], [0, 0, 0, 0], [1, 1, 2, 2], s)
-        assert w["trace_sign"] == 0.0 and w["trace_log"] < -1e300
+    m = ch.model
+    empty = next(s for s in range(m.nsst) if 0 in m.states[s])
+    w = ch.set_config([1.0, 2.0], [0, 0], [0, 0], [2, 1], empty)
+    assert w["trace_sign"] == 0.0 and w["trace_log"] < -1e300
+    full = next(s for s in range(m.nsst) if m.connect[s, 0, 0, 1] >= 0)
+    w = ch.set_config([1.0, 2.0], [0, 0], [0, 0], [2, 1], full)
+    assert w["trace_sign"] != 0.0 and np.isfinite(w["trace_log"])
 
 
 def test_determinant_and_inverse_brute_force():
